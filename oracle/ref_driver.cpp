@@ -1,0 +1,206 @@
+/*
+ * ref_driver.cpp -- drives gala's own compiled C++ (TEST INFRASTRUCTURE ONLY).
+ *
+ * Links, from the reference tree as it lies, potential/potential/src/cpotential.cpp,
+ * potential/hamiltonian/src/chamiltonian.cpp, potential/frame/src/cframe.cpp,
+ * potential/frame/builtin/builtin_frames.cpp and
+ * integrate/cyintegrators/dopri/dop853.cpp (see oracle/Makefile, target "ref").
+ * builtin_potentials.cpp cannot be built here (it includes the header that
+ * gala's setup.py generates, setup.py:523-532), so the potential functions
+ * installed in the reference CPotential table are the oracle's per-component
+ * functions behind adapters with the reference function-pointer signatures
+ * (potential/src/funcdefs.h:3-6).  What this pins: the reference's composite
+ * logic (c_potential/c_gradient/c_hessian/c_d2_dr2 with shift/rotate) and its
+ * DOP853 (dopcor + dense output + Fwrapper_T/hamiltonian_gradient_T), driven
+ * the way dop853.pyx:90-250 drives it (that Cython file cannot be built here).
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "potential/src/cpotential.h"
+#include "frame/src/cframe.h"
+#include "hamiltonian/src/chamiltonian.h"
+#include "dopri/dop853.h"
+#include "gala_oracle.h"
+
+extern void static_frame_gradient(double t, double *pars, double *q, int n_dim, size_t N,
+                                  double *grad, void *state);
+extern double static_frame_hamiltonian(double t, double *pars, double *qp, int n_dim);
+
+namespace {
+
+// The reference passes only pars to a component function; the adapters find
+// the component type from a compile-time tag and rebuild an orc_comp around
+// the parameter pointer.
+template <int TYPE>
+orc_comp as_comp(const double *pars) {
+    orc_comp c;
+    memset(&c, 0, sizeof(c));
+    c.type = TYPE;
+    for (int k = 0; k < 8; k++) c.pars[k] = pars[k];
+    return c;
+}
+
+template <int TYPE>
+double ad_value(double t, double *pars, double *q, int n_dim, void *state) {
+    orc_comp c = as_comp<TYPE>(pars);
+    return orc_component_value(&c, q);
+}
+
+template <int TYPE>
+double ad_density(double t, double *pars, double *q, int n_dim, void *state) {
+    orc_comp c = as_comp<TYPE>(pars);
+    return orc_component_density(&c, q);
+}
+
+template <int TYPE>
+void ad_gradient(double t, double *pars, double *q, int n_dim, size_t N, double *grad,
+                 void *state) {
+    orc_comp c = as_comp<TYPE>(pars);
+    for (size_t i = 0; i < N; i++) {
+        double qi[3] = {q[i], q[N + i], q[2 * N + i]};
+        double g[3] = {grad[i], grad[N + i], grad[2 * N + i]};
+        orc_component_gradient(&c, qi, g);
+        grad[i] = g[0];
+        grad[N + i] = g[1];
+        grad[2 * N + i] = g[2];
+    }
+}
+
+template <int TYPE>
+void ad_hessian(double t, double *pars, double *q, int n_dim, double *hess, void *state) {
+    orc_comp c = as_comp<TYPE>(pars);
+    orc_component_hessian(&c, q, hess);
+}
+
+struct RefPot {
+    CPotential *cp;
+    std::vector<double> store;
+};
+
+template <int TYPE>
+void install(CPotential *cp, int i) {
+    cp->value[i] = &ad_value<TYPE>;
+    cp->density[i] = &ad_density<TYPE>;
+    cp->gradient[i] = &ad_gradient<TYPE>;
+    cp->hessian[i] = &ad_hessian<TYPE>;
+}
+
+// Mirrors CPotentialWrapper.init (cpotential.pyx:57-102) and the composite
+// assembly (ccompositepotential.pyx:27-70): borrowed pars/q0/R per component.
+RefPot *build(const orc_pot *P) {
+    RefPot *rp = new RefPot;
+    rp->store.resize((size_t)P->n * 20);
+    CPotential *cp = allocate_cpotential(P->n);
+    cp->n_dim = 3;
+    cp->null = 0;
+    for (int i = 0; i < P->n; i++) {
+        const orc_comp &c = P->c[i];
+        double *blk = &rp->store[(size_t)i * 20];
+        memcpy(blk, c.pars, 8 * sizeof(double));
+        memcpy(blk + 8, c.q0, 3 * sizeof(double));
+        memcpy(blk + 11, c.R, 9 * sizeof(double));
+        cp->parameters[i] = blk;
+        cp->q0[i] = blk + 8;
+        cp->R[i] = blk + 11;
+        cp->n_params[i] = 8;
+        cp->state[i] = NULL;
+        cp->do_shift_rotate[i] = c.shift_rotate;
+        switch (c.type) {
+        case ORC_HERNQUIST: install<ORC_HERNQUIST>(cp, i); break;
+        case ORC_PLUMMER: install<ORC_PLUMMER>(cp, i); break;
+        case ORC_MIYAMOTONAGAI: install<ORC_MIYAMOTONAGAI>(cp, i); break;
+        case ORC_NFW: install<ORC_NFW>(cp, i); break;
+        default: install<ORC_NULL>(cp, i); break;
+        }
+    }
+    rp->cp = cp;
+    return rp;
+}
+
+void destroy(RefPot *rp) {
+    free_cpotential(rp->cp);
+    delete rp;
+}
+
+}  // namespace
+
+extern "C" {
+
+double ref_value(const orc_pot *P, double t, const double *q) {
+    RefPot *rp = build(P);
+    double v = c_potential(rp->cp, t, const_cast<double *>(q));
+    destroy(rp);
+    return v;
+}
+
+double ref_density(const orc_pot *P, double t, const double *q) {
+    RefPot *rp = build(P);
+    double v = c_density(rp->cp, t, const_cast<double *>(q));
+    destroy(rp);
+    return v;
+}
+
+void ref_gradient(const orc_pot *P, size_t N, double t, const double *q, double *grad) {
+    RefPot *rp = build(P);
+    c_gradient(rp->cp, N, t, const_cast<double *>(q), grad);
+    destroy(rp);
+}
+
+void ref_hessian(const orc_pot *P, double t, const double *q, double *hess) {
+    RefPot *rp = build(P);
+    c_hessian(rp->cp, t, const_cast<double *>(q), hess);
+    destroy(rp);
+}
+
+double ref_d2_dr2(const orc_pot *P, double t, const double *q) {
+    RefPot *rp = build(P);
+    double eps[3];
+    double v = c_d2_dr2(rp->cp, t, const_cast<double *>(q), eps);
+    destroy(rp);
+    return v;
+}
+
+// dop853_integrate_hamiltonian (dop853.pyx:196-250) calling dop853_helper's
+// dop853(...) invocation (dop853.pyx:157-182) per batch of nbatch orbits.
+// w0 (6,N); out (6,ntimes,N) if save_all else (6,N); status[N].
+void ref_dop853(const orc_pot *P, const double *w0, size_t N, const double *t, size_t ntimes,
+                double atol, double rtol, long nmax, long nstiff, double dt_max, int save_all,
+                int nbatch, double *out, int *status) {
+    RefPot *rp = build(P);
+    CFrameType cf;
+    cf.energy = (energyfunc)static_frame_hamiltonian;
+    cf.gradient = (gradientfunc)static_frame_gradient;
+    cf.hessian = NULL;
+    cf.n_params = 0;
+    cf.parameters = NULL;
+    for (size_t i0 = 0; i0 < N; i0 += nbatch) {
+        size_t i1 = i0 + nbatch < N ? i0 + nbatch : N;
+        unsigned nb = (unsigned)(i1 - i0), size = 6 * nb;
+        std::vector<double> w(size), ow(save_all ? size * ntimes : 1);
+        for (int k = 0; k < 6; k++)
+            for (unsigned i = 0; i < nb; i++) w[k * nb + i] = w0[k * N + i0 + i];
+        Dop853DenseState *state = save_all ? dop853_dense_state_alloc(size, size) : NULL;
+        int res = dop853(size, (FcnEqDiff)Fwrapper_T, rp->cp, &cf, nb, 0, NULL, t[0], w.data(),
+                         t[ntimes - 1], &rtol, &atol, 0, NULL, 0, NULL, 2.220446049250313e-16,
+                         0.0, 0.0, 0.0, 0.0, dt_max, t[1] - t[0], nmax, 1, nstiff,
+                         save_all ? size : 0, NULL, 0, state, const_cast<double *>(t),
+                         (int)ntimes, save_all ? ow.data() : NULL);
+        if (state) dop853_dense_state_free(state, size);
+        for (unsigned i = 0; i < nb; i++) {
+            status[i0 + i] = res;
+            for (int k = 0; k < 6; k++) {
+                if (save_all) {
+                    for (size_t j = 0; j < ntimes; j++)
+                        out[(k * ntimes + j) * N + i0 + i] = ow[j * size + k * nb + i];
+                } else {
+                    out[k * N + i0 + i] = w[k * nb + i];
+                }
+            }
+        }
+    }
+    destroy(rp);
+}
+
+}  // extern "C"
