@@ -1,0 +1,416 @@
+// gamd_dop853.hip -- persistent one-lane-per-orbit DOP853 with a work queue.
+//
+// C ABI: gamd_dop853 (include/gala_amd.h).  Restates Hairer's DOP853 as gala
+// runs it (integrate/cyintegrators/dopri/dop853.cpp: dopcor :89-650, front
+// end defaults :653-866, dense output :492-612 and contd8 :869-904), for
+// systems of one orbit each (gala's nbatch=1, dop853.pyx:228-245), with the
+// right-hand side of hamiltonian_gradient_T for the static frame
+// (hamiltonian/src/chamiltonian.cpp:38-57): f(q, p) = (p, -grad Phi(q)).
+//
+// GPU structure:
+//   * one lane owns one orbit; y, k1..k10 and the stage vector live in VGPRs
+//     (about 80 doubles), so an accepted step touches HBM not at all;
+//   * the grid is persistent (a few workgroups per CU) and lanes pull orbits
+//     from a global queue: when a lane's orbit ends, the wave claims new
+//     orbits for all its idle lanes with ONE atomicAdd (ballot + popcount +
+//     mbcnt), so lanes never sit idle while the wave still has work and the
+//     uneven per-orbit step counts do not serialise whole waves;
+//   * the queue hands out orbits in a caller-chosen order (longest first
+//     keeps the tail short); the loop ends when the queue is exhausted for
+//     every lane of the wave, which every wave reaches.
+#include <math.h>
+
+#include "dop853_tableau.h"
+#include "gamd_host.hpp"
+
+using namespace gamd;
+namespace T = gamd::dop;
+
+namespace {
+
+struct DopArgs {
+    const double *w0;
+    int64_t N;
+    const double *t0;
+    const double *t1;
+    const double *h0;
+    double rtol, atol;
+    int64_t nmax, nstiff;
+    double hmax;           // <= 0: |t1 - t0| per orbit
+    const double *t_out;
+    int64_t n_out;
+    double *w_out;
+    double *w_final;
+    int32_t *status;
+    int64_t *stats;
+    const int32_t *order;
+    unsigned long long *queue;
+};
+
+#define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
+
+// f = (p, -grad Phi(q))
+template <bool SCF>
+__device__ __forceinline__ void rhs(const DevPot &P, const double *y, double *f) {
+    double gx, gy, gz;
+    pot_grad<SCF>(P, y[0], y[1], y[2], gx, gy, gz);
+    f[0] = y[3];
+    f[1] = y[4];
+    f[2] = y[5];
+    f[3] = -gx;
+    f[4] = -gy;
+    f[5] = -gz;
+}
+
+__device__ __forceinline__ double sign_d(double a, double b) { return (b < 0.0) ? -fabs(a) : fabs(a); }
+// comparison-based min/max as dop853.cpp:14-16: with a NaN argument they
+// return the second operand, which decides how a NaN orbit ends (-2 after
+// nmax attempts, as in the reference) where fmin/fmax would drop the NaN
+__device__ __forceinline__ double min_d(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double max_d(double a, double b) { return (a > b) ? a : b; }
+
+// Hairer's initial step guess, dop853.cpp:18-86 (scalar tolerances).
+template <bool SCF>
+__device__ double hinit(const DevPot &P, double x, const double *y, double posneg,
+                        const double *f0, double hmax, double rtol, double atol) {
+    double dnf = 0.0, dny = 0.0;
+    FOR6 {
+        const double sk = atol + rtol * fabs(y[c]);
+        double sqr = f0[c] / sk;
+        dnf += sqr * sqr;
+        sqr = y[c] / sk;
+        dny += sqr * sqr;
+    }
+    double h = ((dnf <= 1.0e-10) || (dny <= 1.0e-10)) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+    h = min_d(h, hmax);
+    h = sign_d(h, posneg);
+    double yy[6], f1[6];
+    FOR6 yy[c] = y[c] + h * f0[c];
+    rhs<SCF>(P, yy, f1);
+    double der2 = 0.0;
+    FOR6 {
+        const double sk = atol + rtol * fabs(y[c]);
+        const double sqr = (f1[c] - f0[c]) / sk;
+        der2 += sqr * sqr;
+    }
+    der2 = sqrt(der2) / h;
+    const double der12 = max_d(fabs(der2), sqrt(dnf));
+    const double h1 = (der12 <= 1.0e-15) ? max_d(1.0e-6, fabs(h) * 1.0e-3) : pow(0.01 / der12, 1.0 / 8.0);
+    h = min_d(100.0 * fabs(h), min_d(h1, hmax));
+    return sign_d(h, posneg);
+}
+
+template <bool SCF, bool DENSE>
+__global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a) {
+    const DevPot &P = *Pp;
+    const int lane = threadIdx.x & 63;
+    const double uround = 2.220446049250313e-16;  // np.finfo(float).eps (dop853.pyx:165)
+    const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0, expo1 = 1.0 / 8.0;
+
+    bool active = false, exhausted = false;
+    int64_t orb = 0;
+    double x = 0, xend = 0, h = 0, hmax = 0, posneg = 1, facold = 1e-4, hlamb = 0;
+    int64_t nstep = 0, naccpt = 0, nrejct = 0, nfcn = 0, oidx = 0;
+    int reject = 0, iasti = 0, nonsti = 0;
+    double y[6], k1[6];
+
+    while (true) {
+        // ---- refill idle lanes from the queue (one atomic per wave) ----
+        const bool need = !active && !exhausted;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((long long)want) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(want));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned long long below = want & ((1ull << lane) - 1ull);
+                const unsigned long long slot = base + (unsigned long long)__popcll(below);
+                if (slot < (unsigned long long)a.N) {
+                    orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
+                    if (orb < 0 || orb >= a.N) orb = (int64_t)slot;  // malformed order: fall back
+                    FOR6 y[c] = a.w0[c * a.N + orb];
+                    x = a.t0[orb];
+                    xend = a.t1[orb];
+                    h = a.h0 ? a.h0[orb] : 0.0;
+                    hmax = a.hmax > 0.0 ? a.hmax : fabs(xend - x);  // dop853.cpp:787-788, 312
+                    posneg = sign_d(1.0, xend - x);
+                    facold = 1.0e-4;
+                    hlamb = 0.0;
+                    nstep = naccpt = nrejct = 0;
+                    nfcn = 2;
+                    oidx = 0;
+                    reject = iasti = nonsti = 0;
+                    rhs<SCF>(P, y, k1);
+                    if (h == 0.0) h = hinit<SCF>(P, x, y, posneg, k1, hmax, a.rtol, a.atol);
+                    active = true;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (!active) continue;
+
+        // ---- one step attempt (dopcor basic step, dop853.cpp:339-648) ----
+        int code = 0;  // nonzero: the orbit ends with this idid
+        bool last = false;
+        if (nstep > a.nmax) {
+            code = -2;
+        } else if (0.1 * fabs(h) <= fabs(x) * uround) {
+            code = -3;
+        }
+        if (code == 0) {
+            if ((x + 1.01 * h - xend) * posneg > 0.0) {
+                h = xend - x;
+                last = true;
+            }
+            nstep++;
+            double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], yy[6];
+            FOR6 yy[c] = y[c] + h * T::A[1][0] * k1[c];
+            rhs<SCF>(P, yy, k2);
+            FOR6 yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
+            rhs<SCF>(P, yy, k3);
+            FOR6 yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
+            rhs<SCF>(P, yy, k4);
+            FOR6 yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
+            rhs<SCF>(P, yy, k5);
+            FOR6 yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
+            rhs<SCF>(P, yy, k6);
+            FOR6 yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
+                                     T::A[6][5] * k6[c]);
+            rhs<SCF>(P, yy, k7);
+            FOR6 yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
+                                     T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
+            rhs<SCF>(P, yy, k8);
+            FOR6 yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
+                                     T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
+            rhs<SCF>(P, yy, k9);
+            FOR6 yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
+                                     T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
+                                     T::A[9][8] * k9[c]);
+            rhs<SCF>(P, yy, k10);
+            FOR6 yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
+                                     T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
+                                     T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
+            rhs<SCF>(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
+            const double xph = x + h;
+            FOR6 yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
+                                     T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
+                                     T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
+            rhs<SCF>(P, yy, k3);  // stage 12 reuses k3 (:408)
+            nfcn += 11;
+            FOR6 {
+                k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
+                        T::B[8] * k9[c] + T::B[9] * k10[c] + T::B[10] * k2[c] + T::B[11] * k3[c];
+                k5[c] = y[c] + h * k4[c];
+            }
+            // error estimation (:416-444), n = 6
+            double err = 0.0, err2 = 0.0;
+            FOR6 {
+                const double sk = a.atol + a.rtol * max_d(fabs(y[c]), fabs(k5[c]));
+                double erri = k4[c] - T::BHH[0] * k1[c] - T::BHH[8] * k9[c] - T::BHH[11] * k3[c];
+                double sqr = erri / sk;
+                err2 += sqr * sqr;
+                erri = T::E5[0] * k1[c] + T::E5[5] * k6[c] + T::E5[6] * k7[c] + T::E5[7] * k8[c] +
+                       T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
+                sqr = erri / sk;
+                err += sqr * sqr;
+            }
+            double deno = err + 0.01 * err2;
+            if (deno <= 0.0) deno = 1.0;
+            err = fabs(h) * err * sqrt(1.0 / (deno * 6.0));
+            // step size factor (:447-452), beta = 0
+            const double fac11 = pow(err, expo1);
+            const double fac = max_d(facc2, min_d(facc1, fac11 / safe));
+            double hnew = h / fac;
+
+            if (err <= 1.0) {
+                // accepted (:454-637)
+                facold = max_d(err, 1.0e-4);
+                naccpt++;
+                rhs<SCF>(P, k5, k4);  // FSAL
+                nfcn++;
+                // stiffness test (:463-486); returns -4 on the first detection
+                if (a.nstiff > 0 && (!(naccpt % a.nstiff) || iasti > 0)) {
+                    double stnum = 0.0, stden = 0.0;
+                    FOR6 {
+                        double sqr = k4[c] - k3[c];
+                        stnum += sqr * sqr;
+                        sqr = k5[c] - yy[c];
+                        stden += sqr * sqr;
+                    }
+                    if (stden > 0.0) hlamb = h * sqrt(stnum / stden);
+                    if (hlamb > 6.1) {
+                        nonsti = 0;
+                        iasti++;
+                        code = -4;
+                    } else {
+                        nonsti++;
+                        if (nonsti == 6) iasti = 0;
+                    }
+                }
+                if (code == 0) {
+                    if (DENSE) {
+                        // dense output coefficients (:492-582) and sampling (:584-612)
+                        double r1[6], r2[6], r3[6], r4[6], r5[6], r6[6], r7[6], r8[6];
+                        FOR6 {
+                            r1[c] = y[c];
+                            const double ydiff = k5[c] - y[c];
+                            r2[c] = ydiff;
+                            const double bspl = h * k1[c] - ydiff;
+                            r3[c] = bspl;
+                            r4[c] = ydiff - h * k4[c] - bspl;
+                            r5[c] = T::D[0][0] * k1[c] + T::D[0][5] * k6[c] + T::D[0][6] * k7[c] +
+                                    T::D[0][7] * k8[c] + T::D[0][8] * k9[c] + T::D[0][9] * k10[c] +
+                                    T::D[0][10] * k2[c] + T::D[0][11] * k3[c];
+                            r6[c] = T::D[1][0] * k1[c] + T::D[1][5] * k6[c] + T::D[1][6] * k7[c] +
+                                    T::D[1][7] * k8[c] + T::D[1][8] * k9[c] + T::D[1][9] * k10[c] +
+                                    T::D[1][10] * k2[c] + T::D[1][11] * k3[c];
+                            r7[c] = T::D[2][0] * k1[c] + T::D[2][5] * k6[c] + T::D[2][6] * k7[c] +
+                                    T::D[2][7] * k8[c] + T::D[2][8] * k9[c] + T::D[2][9] * k10[c] +
+                                    T::D[2][10] * k2[c] + T::D[2][11] * k3[c];
+                            r8[c] = T::D[3][0] * k1[c] + T::D[3][5] * k6[c] + T::D[3][6] * k7[c] +
+                                    T::D[3][7] * k8[c] + T::D[3][8] * k9[c] + T::D[3][9] * k10[c] +
+                                    T::D[3][10] * k2[c] + T::D[3][11] * k3[c];
+                        }
+                        FOR6 yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
+                                                 T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
+                                                 T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
+                                                 T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
+                        rhs<SCF>(P, yy, k10);
+                        FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
+                                                 T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
+                                                 T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
+                                                 T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
+                        rhs<SCF>(P, yy, k2);
+                        FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
+                                                 T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
+                                                 T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
+                                                 T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
+                        rhs<SCF>(P, yy, k3);
+                        nfcn += 3;
+                        FOR6 {
+                            r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
+                                         T::D[0][14] * k2[c] + T::D[0][15] * k3[c]);
+                            r6[c] = h * (r6[c] + T::D[1][12] * k4[c] + T::D[1][13] * k10[c] +
+                                         T::D[1][14] * k2[c] + T::D[1][15] * k3[c]);
+                            r7[c] = h * (r7[c] + T::D[2][12] * k4[c] + T::D[2][13] * k10[c] +
+                                         T::D[2][14] * k2[c] + T::D[2][15] * k3[c]);
+                            r8[c] = h * (r8[c] + T::D[3][12] * k4[c] + T::D[3][13] * k10[c] +
+                                         T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
+                        }
+                        const double x0 = x, x1 = x + h;
+                        const int64_t so = a.n_out * a.N;
+                        while (oidx < a.n_out) {
+                            const double to = a.t_out[oidx];
+                            if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
+                                const double s = (to - x) / h, s1 = 1.0 - s;
+                                FOR6 a.w_out[c * so + oidx * a.N + orb] =
+                                    r1[c] + s * (r2[c] + s1 * (r3[c] + s * (r4[c] + s1 * (r5[c] +
+                                    s * (r6[c] + s1 * (r7[c] + s * r8[c]))))));
+                                oidx++;
+                            } else {
+                                break;
+                            }
+                        }
+                    }
+                    FOR6 {
+                        k1[c] = k4[c];
+                        y[c] = k5[c];
+                    }
+                    x = xph;
+                    if (last) {
+                        code = 1;
+                    } else {
+                        if (fabs(hnew) > hmax) hnew = posneg * hmax;
+                        if (reject) hnew = posneg * min_d(fabs(hnew), fabs(h));
+                        reject = 0;
+                    }
+                }
+            } else {
+                // rejected (:638-645)
+                hnew = h / min_d(facc1, fac11 / safe);
+                reject = 1;
+                if (naccpt >= 1) nrejct++;
+            }
+            if (code == 0) h = hnew;
+        }
+        if (code != 0) {
+            FOR6 a.w_final[c * a.N + orb] = y[c];
+            if (a.status) a.status[orb] = code;
+            if (a.stats) {
+                a.stats[4 * orb + 0] = nstep;
+                a.stats[4 * orb + 1] = naccpt;
+                a.stats[4 * orb + 2] = nrejct;
+                a.stats[4 * orb + 3] = nfcn;
+            }
+            active = false;
+        }
+    }
+}
+
+template <bool SCF, bool DENSE>
+int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
+    auto kern = k_dop853<SCF, DENSE>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    int64_t blocks = (int64_t)per_cu * cu_count();
+    const int64_t needed = (a.N + block - 1) / block;
+    if (blocks > needed) blocks = needed;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, s, P, a);
+    return launch_status();
+}
+
+}  // namespace
+
+extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
+                           const double *t1, const double *h0, double rtol, double atol,
+                           int64_t nmax, int64_t nstiff, double hmax, const double *t_out,
+                           size_t n_out, double *w_out, double *w_final, int32_t *status,
+                           int64_t *stats, const int32_t *order, int block, void *stream) {
+    if (!pot || !w0 || !t0 || !t1 || !w_final || (w_out && (!t_out || n_out == 0)))
+        return GAMD_ERR_INVALID;
+    if (block == 0) block = 256;
+    if (block < 64 || block > 1024 || block % 64) return GAMD_ERR_INVALID;
+    if (nmax == 0) nmax = 1000000;                   // dop853.cpp:673-674
+    else if (nmax < 0) return GAMD_ERR_INVALID;      // :675-679
+    if (nstiff == 0) nstiff = 1000;                  // :691-692
+    else if (nstiff < 0) nstiff = 0;                 // :693-694 (never tested)
+    if (N == 0) return GAMD_OK;
+    hipStream_t s = as_stream(stream);
+    unsigned long long *queue = nullptr;
+    GAMD_HIP_CHECK(hipMallocAsync((void **)&queue, sizeof(unsigned long long), s));
+    GAMD_HIP_CHECK(hipMemsetAsync(queue, 0, sizeof(unsigned long long), s));
+    DopArgs a;
+    a.w0 = w0;
+    a.N = (int64_t)N;
+    a.t0 = t0;
+    a.t1 = t1;
+    a.h0 = h0;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.nmax = nmax;
+    a.nstiff = nstiff;
+    a.hmax = hmax;
+    a.t_out = t_out;
+    a.n_out = (int64_t)n_out;
+    a.w_out = w_out;
+    a.w_final = w_final;
+    a.status = status;
+    a.stats = stats;
+    a.order = order;
+    a.queue = queue;
+    int rc;
+    const bool scf = pot->dev.has_scf;
+    if (w_out)
+        rc = scf ? launch<true, true>(pot->ddev, a, block, s) : launch<false, true>(pot->ddev, a, block, s);
+    else
+        rc = scf ? launch<true, false>(pot->ddev, a, block, s) : launch<false, false>(pot->ddev, a, block, s);
+    hipError_t fe = hipFreeAsync(queue, s);
+    if (rc == GAMD_OK && fe != hipSuccess) rc = GAMD_ERR_HIP;
+    return rc;
+}

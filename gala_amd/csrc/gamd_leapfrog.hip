@@ -1,0 +1,109 @@
+// gamd_leapfrog.hip -- one-lane-per-orbit FP64 leapfrog.
+//
+// C ABI: gamd_leapfrog (include/gala_amd.h).  Restates
+// leapfrog_integrate_hamiltonian (integrate/cyintegrators/leapfrog.pyx:54-121):
+// a half kick from the initial gradient (c_init_velocity, :24-32), then per
+// step drift x += v_half dt, gradient at the new x, synchronised
+// v = v_half - g dt/2 and v_half -= g dt (c_leapfrog_step, :35-51).
+//
+// Each lane keeps its orbit's 9 doubles (x, v, v_half) in VGPRs for the
+// whole integration and touches HBM only to read w0 and write saved
+// states: the (6, nsave, N) layout makes every store a coalesced 512-byte
+// wave store.  The time loop runs inside the kernel, so there is one launch
+// per integration, not one per step.
+#include "gamd_host.hpp"
+
+using namespace gamd;
+
+namespace {
+
+template <bool SCF, bool SAVE>
+__global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp, const double *__restrict__ w0,
+                                                  int64_t N, double dt, int64_t n_steps,
+                                                  const int64_t *__restrict__ nsteps_i,
+                                                  int64_t stride, int64_t nsave,
+                                                  double *__restrict__ wout,
+                                                  double *__restrict__ wfinal) {
+    const DevPot &P = *Pp;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double x = w0[i], y = w0[N + i], z = w0[2 * N + i];
+    double vx = w0[3 * N + i], vy = w0[4 * N + i], vz = w0[5 * N + i];
+    const int64_t ns = nsteps_i ? nsteps_i[i] : n_steps;
+    if (SAVE) {
+        const int64_t s = (int64_t)nsave * N;
+        wout[i] = x;
+        wout[s + i] = y;
+        wout[2 * s + i] = z;
+        wout[3 * s + i] = vx;
+        wout[4 * s + i] = vy;
+        wout[5 * s + i] = vz;
+    }
+    double gx, gy, gz;
+    pot_grad<SCF>(P, x, y, z, gx, gy, gz);
+    double hx = vx - gx * dt / 2.0, hy = vy - gy * dt / 2.0, hz = vz - gz * dt / 2.0;
+    int64_t next_save = stride;
+    for (int64_t j = 1; j <= ns; j++) {
+        x = x + hx * dt;
+        y = y + hy * dt;
+        z = z + hz * dt;
+        pot_grad<SCF>(P, x, y, z, gx, gy, gz);
+        vx = hx - gx * dt / 2.0;
+        vy = hy - gy * dt / 2.0;
+        vz = hz - gz * dt / 2.0;
+        hx = hx - gx * dt;
+        hy = hy - gy * dt;
+        hz = hz - gz * dt;
+        if (SAVE && j == next_save) {
+            next_save += stride;
+            const int64_t s = (int64_t)nsave * N, o = (j / stride) * N + i;
+            wout[o] = x;
+            wout[s + o] = y;
+            wout[2 * s + o] = z;
+            wout[3 * s + o] = vx;
+            wout[4 * s + o] = vy;
+            wout[5 * s + o] = vz;
+        }
+    }
+    if (wfinal) {
+        wfinal[i] = x;
+        wfinal[N + i] = y;
+        wfinal[2 * N + i] = z;
+        wfinal[3 * N + i] = vx;
+        wfinal[4 * N + i] = vy;
+        wfinal[5 * N + i] = vz;
+    }
+}
+
+constexpr unsigned LF_BLOCK = 64;  // one wave per workgroup spreads small N over all CUs
+
+}  // namespace
+
+extern "C" int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t N, double t0,
+                             double dt, int64_t n_steps, const int64_t *n_steps_per_orbit,
+                             int64_t save_stride, double *w_out, double *w_final, void *stream) {
+    if (!pot || !w0 || n_steps < 0 || (w_out && save_stride <= 0) || (!w_out && !w_final))
+        return GAMD_ERR_INVALID;
+    if (w_out && n_steps_per_orbit) return GAMD_ERR_INVALID;  // saved grids need one length
+    if (N == 0) return GAMD_OK;
+    const int64_t nsave = w_out ? n_steps / save_stride + 1 : 0;
+    const unsigned g = grid_for(N, LF_BLOCK);
+    hipStream_t s = as_stream(stream);
+    const bool scf = pot->dev.has_scf;
+    if (w_out) {
+        if (scf)
+            hipLaunchKernelGGL((k_leapfrog<true, true>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
+                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
+        else
+            hipLaunchKernelGGL((k_leapfrog<false, true>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
+                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
+    } else {
+        if (scf)
+            hipLaunchKernelGGL((k_leapfrog<true, false>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
+                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
+        else
+            hipLaunchKernelGGL((k_leapfrog<false, false>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
+                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
+    }
+    return launch_status();
+}

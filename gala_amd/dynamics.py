@@ -1,0 +1,416 @@
+"""Phase-space containers, orbits and mock stellar streams.
+
+Mirrors (gala src/gala/...):
+  PhaseSpacePosition     dynamics/core.py (from_w :528-567)
+  Orbit                  dynamics/orbit.py:22-389 (layout :74-175, energy :361-389)
+  MockStream             dynamics/mockstream/core.py
+  FardalStreamDF         dynamics/mockstream/df.pyx:125-238 (sample), 363-456
+                         (_sample), 61-106 (get_rj_vj_R, transform_from_sat)
+  MockStreamGenerator    dynamics/mockstream/mockstream_generator.py:19-372
+  DirectNBody            dynamics/nbody/core.py (the progenitor-only case)
+
+Arrays are in the unit system of the potential (kpc, Myr, Msun by default):
+positions (3, ...), velocities (3, ...) in kpc/Myr, times in Myr.  Only the
+static frame exists here, so frame transformations are identities.
+"""
+import numpy as np
+
+from . import _device as dev
+from . import _lib
+from .integrate import (DOPRI853Integrator, LeapfrogIntegrator, get_integrator, leapfrog,
+                        parse_time_specification)
+from .potential import Hamiltonian, NullPotential, PotentialBase, StaticFrame
+
+__all__ = ["PhaseSpacePosition", "Orbit", "MockStream", "FardalStreamDF", "StreaklineStreamDF",
+           "MockStreamGenerator", "DirectNBody"]
+
+
+class PhaseSpacePosition:
+    """Positions and velocities, each shaped (3, ...) (dynamics/core.py)."""
+
+    def __init__(self, pos, vel, frame=None):
+        self.xyz = np.asarray(pos, dtype=np.float64)
+        self.v_xyz = np.asarray(vel, dtype=np.float64)
+        if self.xyz.shape[0] != 3 or self.v_xyz.shape != self.xyz.shape:
+            raise ValueError("pos and vel must both have shape (3, ...)")
+        self.frame = frame
+
+    pos = property(lambda self: self.xyz)
+    vel = property(lambda self: self.v_xyz)
+    ndim = 3
+
+    @property
+    def shape(self):
+        return self.xyz.shape[1:]
+
+    def w(self, units=None):
+        """(6, ...) stack of positions and velocities."""
+        return np.vstack((self.xyz[None], self.v_xyz[None])).reshape((6,) + self.shape)
+
+    @classmethod
+    def from_w(cls, w, units=None, frame=None):
+        w = np.asarray(w)
+        return cls(pos=w[:3], vel=w[3:], frame=frame)
+
+    def energy(self, hamiltonian):
+        H = Hamiltonian(hamiltonian)
+        return H.energy(self.w())
+
+    def kinetic_energy(self):
+        return 0.5 * np.sum(self.v_xyz ** 2, axis=0)
+
+    def potential_energy(self, potential):
+        return potential.energy(self.xyz)
+
+    def __getitem__(self, idx):
+        if not isinstance(idx, tuple):
+            idx = (idx,)
+        return type(self)(self.xyz[(slice(None),) + idx], self.v_xyz[(slice(None),) + idx],
+                          frame=self.frame)
+
+    def __len__(self):
+        return self.shape[0] if self.shape else 0
+
+
+class Orbit(PhaseSpacePosition):
+    """A set of orbits: t (ntimes,), pos/vel (3, ntimes[, norbits])
+    (dynamics/orbit.py:74-106)."""
+
+    def __init__(self, pos, vel, t=None, hamiltonian=None, potential=None, frame=None):
+        super().__init__(pos, vel, frame=frame)
+        if t is not None:
+            t = np.atleast_1d(np.asarray(t, dtype=np.float64))
+            if self.xyz.ndim < 2 or t.shape[0] != self.xyz.shape[1]:
+                raise ValueError("t must match the time axis (axis 1) of pos")
+        self.t = t
+        if hamiltonian is not None:
+            self.hamiltonian = Hamiltonian(hamiltonian)
+            self.potential = self.hamiltonian.potential
+            self.frame = self.hamiltonian.frame
+        else:
+            self.hamiltonian = Hamiltonian(potential, frame) if potential is not None else None
+            self.potential = potential
+
+    @classmethod
+    def from_w(cls, w, units=None, t=None, hamiltonian=None, **kw):
+        w = np.asarray(w)
+        return cls(pos=w[:3], vel=w[3:], t=t, hamiltonian=hamiltonian)
+
+    @property
+    def ntimes(self):
+        return self.xyz.shape[1]
+
+    @property
+    def norbits(self):
+        return 1 if self.xyz.ndim == 2 else self.xyz.shape[2]
+
+    def energy(self, hamiltonian=None):
+        """Total energy per unit mass (orbit.py:361-389)."""
+        if self.hamiltonian is None and hamiltonian is None:
+            raise ValueError("To compute the total energy, a hamiltonian object must be provided!")
+        H = self.hamiltonian if hamiltonian is None else Hamiltonian(hamiltonian)
+        return H.energy(self.w())
+
+    def __getitem__(self, idx):
+        if not isinstance(idx, tuple):
+            idx = (idx,)
+        t = self.t
+        if t is not None and len(idx) >= 1:
+            t = t[idx[0]]
+        pos = self.xyz[(slice(None),) + idx]
+        vel = self.v_xyz[(slice(None),) + idx]
+        if t is None or np.ndim(t) == 0:
+            return PhaseSpacePosition(pos, vel, frame=self.frame)
+        return Orbit(pos, vel, t=t, hamiltonian=self.hamiltonian)
+
+
+class MockStream(PhaseSpacePosition):
+    """Stream particles with release times and 't'/'l' tail labels
+    (dynamics/mockstream/core.py)."""
+
+    def __init__(self, pos, vel, release_time=None, lead_trail=None, frame=None):
+        super().__init__(pos, vel, frame=frame)
+        self.release_time = None if release_time is None else np.asarray(release_time)
+        self.lead_trail = None if lead_trail is None else np.asarray(lead_trail)
+
+
+# ---------------------------------------------------------------------------
+# stream distribution functions
+
+class BaseStreamDF:
+    """Release geometry shared by the stream DFs (df.pyx:32-123)."""
+
+    def __init__(self, lead=True, trail=True, random_state=None):
+        self._lead = int(lead)
+        self._trail = int(trail)
+        if random_state is None:
+            random_state = np.random.RandomState()
+        self.random_state = random_state
+        if not self.lead and not self.trail:
+            raise ValueError("You must generate either leading or trailing tails (or both!)")
+
+    lead = property(lambda self: self._lead)
+    trail = property(lambda self: self._trail)
+
+    @staticmethod
+    def _rj_vj_R(potential, G, prog_x, prog_v, prog_m, prog_t):
+        """get_rj_vj_R (df.pyx:61-92) for every release time at once.
+        prog_x, prog_v (ntimes, 3).  Returns rj, vj (ntimes,), R (ntimes, 3, 3)
+        (rows: x_hat, -(x_hat x L_hat), L_hat)."""
+        x0, x1, x2 = prog_x[:, 0], prog_x[:, 1], prog_x[:, 2]
+        v0, v1, v2 = prog_v[:, 0], prog_v[:, 1], prog_v[:, 2]
+        dist = np.sqrt(x0 * x0 + x1 * x1 + x2 * x2)
+        L = np.stack([x1 * v2 - x2 * v1, -x0 * v2 + x2 * v0, x0 * v1 - x1 * v0], axis=1)
+        Lnorm = np.sqrt(L[:, 0] * L[:, 0] + L[:, 1] * L[:, 1] + L[:, 2] * L[:, 2])
+        R = np.empty((len(dist), 3, 3))
+        R[:, 0, :] = prog_x / dist[:, None]
+        R[:, 2, :] = L / Lnorm[:, None]
+        Om = Lnorm / (dist * dist)
+        # c_d2_dr2 of the external potential at every release point, one launch
+        d2r = potential.d2_dr2(np.ascontiguousarray(prog_x.T)) if len(dist) else np.zeros(0)
+        rj = (G * prog_m / (Om * Om - d2r)) ** (1 / 3.0)
+        vj = Om * rj
+        a, b = R[:, 0, :], R[:, 2, :]
+        R[:, 1, 0] = -(a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1])
+        R[:, 1, 1] = -(-a[:, 0] * b[:, 2] + a[:, 2] * b[:, 0])
+        R[:, 1, 2] = -(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
+        return rj, vj, R
+
+    @staticmethod
+    def _from_sat(R, tx, tv, px, pv):
+        """transform_from_sat (df.pyx:94-106): R^T x + prog, per particle."""
+        out_x = (R[:, 0, :] * tx[:, 0:1] + R[:, 1, :] * tx[:, 1:2]) + R[:, 2, :] * tx[:, 2:3] + px
+        out_v = (R[:, 0, :] * tv[:, 0:1] + R[:, 1, :] * tv[:, 1:2]) + R[:, 2, :] * tv[:, 2:3] + pv
+        return out_x, out_v
+
+    def sample(self, prog_orbit, prog_mass, hamiltonian=None, release_every=1, n_particles=1):
+        """Stream-particle initial conditions and release times (df.pyx:125-238)."""
+        if getattr(prog_orbit, "hamiltonian", None) is not None:
+            H = prog_orbit.hamiltonian
+        elif hamiltonian is not None:
+            H = Hamiltonian(hamiltonian)
+        else:
+            raise ValueError("a hamiltonian is needed to sample the stream DF")
+        prog_x = np.ascontiguousarray(prog_orbit.xyz.T)
+        prog_v = np.ascontiguousarray(prog_orbit.v_xyz.T)
+        prog_t = np.asarray(prog_orbit.t, dtype=np.float64)
+        prog_m = np.squeeze(np.asarray(prog_mass, dtype=np.float64))
+        if prog_m.shape == ():
+            prog_m = np.full_like(prog_t, prog_m)
+        if np.iterable(n_particles):
+            n_particles = np.array(n_particles).astype("i4")
+            if not len(n_particles) == len(prog_t):
+                raise ValueError("If passing in an array n_particles, its shape must match the "
+                                 "number of timesteps in the progenitor orbit.")
+        else:
+            N = int(n_particles)
+            n_particles = np.zeros_like(prog_t, dtype="i4")
+            n_particles[::release_every] = N
+        x, v, t1 = self._sample(H.potential, prog_x, prog_v, prog_t, prog_m, n_particles)
+        lt = np.empty(len(t1), dtype="U1")
+        i = 0
+        for n in n_particles:
+            if self._trail:
+                lt[i:i + n] = "t"
+                i += n
+            if self._lead:
+                lt[i:i + n] = "l"
+                i += n
+        return MockStream(pos=x.T, vel=v.T, release_time=t1, lead_trail=lt,
+                          frame=StaticFrame(H.units))
+
+
+class FardalStreamDF(BaseStreamDF):
+    """Fardal+2015 particle release (df.pyx:314-456), gala_modified
+    parameters by default (kx ~ N(2, 0.5), z ~ N(0, 0.5), vt ~ N(0.3, 0.5) kx,
+    vz ~ N(0, 0.5) in units of r_J, v_J)."""
+
+    def __init__(self, gala_modified=None, lead=True, trail=True, random_state=None):
+        super().__init__(lead=lead, trail=trail, random_state=random_state)
+        if gala_modified is None:
+            gala_modified = True
+        self._gala_modified = int(gala_modified)
+
+    def _draws(self, n):
+        """n standard normals, in the order the reference's successive
+        random_state.normal(mean, sd) calls consume them; normal(mean, sd) is
+        mean + sd * z for both RandomState and Generator."""
+        return np.asarray(self.random_state.standard_normal(n), dtype=np.float64)
+
+    def _sample(self, potential, prog_x, prog_v, prog_t, prog_m, nparticles):
+        kd0 = 0.5 if self._gala_modified else 0.4
+        kd4 = 0.5 if self._gala_modified else 0.4
+        use = prog_m != 0
+        idx = np.nonzero(use)[0]
+        tails = [s for s, on in ((1.0, self._trail), (-1.0, self._lead)) if on]
+        rj, vj, R = self._rj_vj_R(potential, potential.G, prog_x[idx], prog_v[idx], prog_m[idx],
+                                  prog_t[idx])
+        counts = np.asarray(nparticles)[idx]
+        # particle order: per release time, trailing block then leading block
+        rel = np.repeat(np.arange(len(idx)), counts * len(tails))
+        sign = np.concatenate([np.repeat(np.array(tails), c) for c in counts]) if len(idx) else np.zeros(0)
+        n = len(rel)
+        z = self._draws(4 * n).reshape(n, 4)
+        kx = 2.0 + kd0 * z[:, 0]
+        srj = sign * rj[rel]
+        svj = sign * vj[rel]
+        tx = np.zeros((n, 3))
+        tv = np.zeros((n, 3))
+        tx[:, 0] = kx * srj
+        tx[:, 2] = (0.0 + 0.5 * z[:, 1]) * srj
+        tv[:, 1] = (0.3 + kd4 * z[:, 2]) * svj
+        if self._gala_modified:
+            tv[:, 1] *= kx
+        tv[:, 2] = (0.0 + 0.5 * z[:, 3]) * svj
+        x, v = self._from_sat(R[rel], tx, tv, prog_x[idx][rel], prog_v[idx][rel])
+        return x, v, prog_t[idx][rel]
+
+
+class StreaklineStreamDF(BaseStreamDF):
+    """Streakline release at +-r_J with +-v_J (df.pyx:240-311)."""
+
+    def _sample(self, potential, prog_x, prog_v, prog_t, prog_m, nparticles):
+        idx = np.nonzero(prog_m != 0)[0]
+        tails = [s for s, on in ((1.0, self._trail), (-1.0, self._lead)) if on]
+        rj, vj, R = self._rj_vj_R(potential, potential.G, prog_x[idx], prog_v[idx], prog_m[idx],
+                                  prog_t[idx])
+        counts = np.asarray(nparticles)[idx]
+        rel = np.repeat(np.arange(len(idx)), counts * len(tails))
+        sign = np.concatenate([np.repeat(np.array(tails), c) for c in counts]) if len(idx) else np.zeros(0)
+        n = len(rel)
+        tx = np.zeros((n, 3))
+        tv = np.zeros((n, 3))
+        tx[:, 0] = sign * rj[rel]
+        tv[:, 1] = sign * vj[rel]
+        x, v = self._from_sat(R[rel], tx, tv, prog_x[idx][rel], prog_v[idx][rel])
+        return x, v, prog_t[idx][rel]
+
+
+# ---------------------------------------------------------------------------
+
+class DirectNBody:
+    """Massive bodies in an external potential (dynamics/nbody/core.py).  Only
+    the case the stream generator builds is supported: one massive body (the
+    progenitor), whose own potential does not act on itself, so its orbit is a
+    test-particle orbit in the external potential (leapfrog.pyx:126-257 with
+    nbody_i == 0 skipped, cpotential.cpp:430)."""
+
+    def __init__(self, w0, particle_potentials, external_potential=None, frame=None, units=None,
+                 save_all=True):
+        if isinstance(w0, PhaseSpacePosition):
+            w = w0.w()
+        else:
+            w = np.asarray(w0, dtype=np.float64)
+        self.w0 = w.reshape(6, -1)
+        if self.w0.shape[1] != 1:
+            raise NotImplementedError("only a single massive body (the progenitor) is supported")
+        self.particle_potentials = [NullPotential() if p is None else p
+                                    for p in particle_potentials]
+        self.external_potential = external_potential
+        self.H = Hamiltonian(external_potential, frame)
+        self.units = self.H.units
+        self.save_all = save_all
+
+    def integrate_orbit(self, Integrator=None, Integrator_kwargs=None, **time_spec):
+        Integrator = get_integrator(Integrator or LeapfrogIntegrator)
+        if Integrator is not LeapfrogIntegrator:
+            raise NotImplementedError("only Leapfrog N-body integration is supported")
+        t = parse_time_specification(self.units, **time_spec)
+        wout, wfin = leapfrog(self.external_potential, self.w0, t[1] - t[0], len(t) - 1,
+                              save=self.save_all)
+        if self.save_all:
+            w = dev.to_host(wout)  # (6, ntimes, 1)
+            return Orbit(pos=w[:3], vel=w[3:], t=t, hamiltonian=self.H)
+        w = dev.to_host(wfin)
+        return PhaseSpacePosition(w[:3], w[3:])
+
+
+class MockStreamGenerator:
+    """Generate a mock stream (mockstream_generator.py:19-372).  Leapfrog
+    runs the release-and-integrate loop in the HIP kernel
+    gamd_mockstream_leapfrog; the stream DF runs on the host with one GPU
+    launch for the tidal-radius derivatives."""
+
+    def __init__(self, df, hamiltonian, progenitor_potential=None):
+        if not isinstance(df, BaseStreamDF):
+            raise TypeError("The input distribution function (DF) instance must be an instance "
+                            f"of a subclass of BaseStreamDF, not {type(df)}.")
+        self.df = df
+        self.hamiltonian = Hamiltonian(hamiltonian)
+        if progenitor_potential is not None and not isinstance(progenitor_potential,
+                                                               PotentialBase):
+            raise TypeError("If specified, the progenitor_potential must be a potential instance.")
+        self.self_gravity = progenitor_potential is not None
+        self.progenitor_potential = progenitor_potential
+
+    def run(self, prog_w0, prog_mass, nbody=None, release_every=1, n_particles=1,
+            output_every=None, output_filename=None, check_filesize=True, overwrite=False,
+            progress=False, Integrator=None, Integrator_kwargs=None, **time_spec):
+        """Returns (stream MockStream, progenitor PhaseSpacePosition)."""
+        if nbody is not None:
+            raise NotImplementedError("extra massive perturbers (nbody=) are not supported")
+        if output_every is not None:
+            raise NotImplementedError("snapshot output (output_every=) is not supported")
+        Integrator = get_integrator(Integrator or DOPRI853Integrator)
+        if Integrator is not LeapfrogIntegrator:
+            raise NotImplementedError("mock streams are generated with Integrator='leapfrog' "
+                                      "only; DOP853 streams are not implemented")
+        units = self.hamiltonian.units
+        pot = self.hamiltonian.potential
+        t = parse_time_specification(units, **time_spec)
+        if isinstance(prog_w0, PhaseSpacePosition):
+            pw0 = prog_w0.w().reshape(6, 1)
+        else:
+            pw0 = np.asarray(prog_w0, dtype=np.float64).reshape(6, 1)
+        # progenitor orbit over t (backward if dt < 0), then reversed
+        wout, _ = leapfrog(pot, pw0, t[1] - t[0], len(t) - 1, save=True)
+        w = dev.to_host(wout)[:, :, 0]
+        orbit_t = t
+        if t[1] < t[0]:
+            w = w[:, ::-1]
+            orbit_t = t[::-1]
+        prog_orbit = Orbit(pos=w[:3], vel=w[3:], t=orbit_t, hamiltonian=self.hamiltonian)
+        stream_w0 = self.df.sample(prog_orbit, prog_mass, hamiltonian=self.hamiltonian,
+                                   release_every=release_every, n_particles=n_particles)
+        # release bookkeeping (mockstream_generator.py:272-283)
+        unq_t1s, nstream = np.unique(stream_w0.release_time, return_counts=True)
+        all_nstream = np.zeros(len(orbit_t), dtype=int)
+        for t1, n in zip(unq_t1s, nstream):
+            all_nstream[np.isclose(orbit_t, t1)] = n
+        # the forward N-body trajectory of the progenitor (mockstream.pyx:536-549)
+        dt = orbit_t[1] - orbit_t[0]
+        nfull = len(orbit_t)
+        traj_dev, _ = leapfrog(pot, np.ascontiguousarray(w[:, 0:1]), dt, nfull - 1, save=True)
+        traj_dev = traj_dev[:, :, 0].contiguous()
+        w0s = np.ascontiguousarray(stream_w0.w())
+        n = w0s.shape[1]
+        rel_t = np.asarray(stream_w0.release_time, dtype=np.float64)
+        out = dev.empty((6, n))
+        lib = _lib.load()
+        prog_handle = self.progenitor_potential.c_handle if self.self_gravity else None
+        w0d = dev.to_dev(w0s)
+        reld = dev.to_dev(rel_t)
+        tfinal = float(orbit_t[-1])
+        _lib.check(lib.gamd_mockstream_leapfrog(pot.c_handle, prog_handle, traj_dev.data_ptr(),
+                                                nfull, float(orbit_t[0]), float(dt),
+                                                w0d.data_ptr(), reld.data_ptr(), n, tfinal,
+                                                out.data_ptr(), dev.stream_ptr()),
+                   "gamd_mockstream_leapfrog")
+        raw_stream = dev.to_host(out)
+        traj = dev.to_host(traj_dev)
+        # the progenitor state the reference returns: that of the last release
+        # batch, integrated from its release state to tfinal
+        nz = np.nonzero(all_nstream)[0]
+        last_idx = int(nz[-1]) if len(nz) else nfull - 1
+        if last_idx == nfull - 1:
+            prog_final = traj[:, -1]
+        else:
+            _, pf = leapfrog(pot, np.ascontiguousarray(traj[:, last_idx:last_idx + 1]), dt,
+                             nfull - 1 - last_idx, save=False)
+            prog_final = dev.to_host(pf)[:, 0]
+        stream = MockStream(pos=raw_stream[:3], vel=raw_stream[3:],
+                            release_time=stream_w0.release_time,
+                            lead_trail=stream_w0.lead_trail, frame=self.hamiltonian.frame)
+        prog = PhaseSpacePosition(pos=prog_final[:3, None], vel=prog_final[3:, None],
+                                  frame=self.hamiltonian.frame)
+        return stream, prog

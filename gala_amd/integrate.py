@@ -1,0 +1,226 @@
+"""Time grids, integrator lookup and the orbit-integration dispatch.
+
+Mirrors gala's src/gala/integrate/timespec.py:10-147 (parse_time_specification),
+integrate/lookup.py:14-84 (get_integrator) and the Cython-path dispatch of
+Hamiltonian.integrate_orbit (potential/hamiltonian/chamiltonian.pyx:274-380).
+The integrations themselves run in the HIP kernels (gamd_leapfrog,
+gamd_dop853).  Lower-level functions taking device tensors (``leapfrog``,
+``dop853``) serve callers whose inputs already live in HBM.
+"""
+import numpy as np
+
+from . import _device as dev
+from . import _lib
+
+__all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
+           "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "dop853",
+           "integrate_hamiltonian"]
+
+
+class Integrator:
+    """Integrator name tags (gala's classes live in integrate/pyintegrators;
+    here they only select the HIP kernel)."""
+    name = None
+
+
+class LeapfrogIntegrator(Integrator):
+    name = "leapfrog"
+
+
+class DOPRI853Integrator(Integrator):
+    name = "dopri853"
+
+
+class Ruth4Integrator(Integrator):
+    name = "ruth4"
+
+
+class RK5Integrator(Integrator):
+    name = "rk5"
+
+
+_integrator_name_mapping = {
+    "leapfrog": LeapfrogIntegrator,
+    "dopri853": DOPRI853Integrator,
+    "dop853": DOPRI853Integrator,
+    "rk5": RK5Integrator,
+    "ruth4": Ruth4Integrator,
+}
+
+
+def get_integrator(name):
+    """Integrator class from a (case-insensitive) name or class (lookup.py:23-84)."""
+    if not isinstance(name, str):
+        if name not in list(_integrator_name_mapping.values()):
+            raise ValueError(f"Integrator class '{name}' is not recognized. Valid classes are: "
+                             f"{list(_integrator_name_mapping.values())}")
+        return name
+    try:
+        return _integrator_name_mapping[name.lower()]
+    except KeyError as e:
+        raise ValueError(f"Integrator name '{name}' is not recognized. Valid names are: "
+                         f"{list(_integrator_name_mapping.keys())}") from e
+
+
+def parse_time_specification(units=None, dt=None, n_steps=None, t1=None, t2=None, t=None):
+    """Array of times from one of: (dt, n_steps[, t1]), (dt, t1, t2),
+    (dt array, t1), (n_steps, t1, t2) or t (timespec.py:10-147)."""
+    if n_steps is not None:
+        n_steps = int(n_steps)
+    if t is not None:
+        return np.asarray(t).astype(np.float64)
+    if dt is None and (t1 is None or t2 is None or n_steps is None):
+        raise ValueError("Invalid specification of integration time. See docstring for more "
+                         "information.")
+    if dt is not None and n_steps is not None:
+        if t1 is None:
+            t1 = 0.0
+        return parse_time_specification(units, dt=np.ones(n_steps + 1) * dt, t1=t1)
+    if dt is not None and t1 is not None and t2 is not None:
+        if t2 < t1 and dt < 0:
+            t_i, times, ii = t1, [], 0
+            while (t_i > t2) and (ii < 1e6):
+                times.append(t_i)
+                t_i += dt
+            if times[-1] != t2:
+                times.append(t2)
+            return np.array(times, dtype=np.float64)
+        if t2 > t1 and dt > 0:
+            t_i, times, ii = t1, [], 0
+            while (t_i < t2) and (ii < 1e6):
+                times.append(t_i)
+                t_i += dt
+            return np.array(times, dtype=np.float64)
+        if dt == 0:
+            raise ValueError("dt must be non-zero.")
+        raise ValueError("If t2 < t1, dt must be negative. If t1 < t2, dt must be positive.")
+    if isinstance(dt, np.ndarray) and t1 is not None:
+        times = np.cumsum(np.append([0.0], dt)) + t1
+        return times[:-1].astype(np.float64)
+    if dt is None and not (t1 is None or t2 is None or n_steps is None):
+        return np.linspace(t1, t2, n_steps, endpoint=True).astype(np.float64)
+    raise ValueError("Invalid options. See docstring.")
+
+
+# ---------------------------------------------------------------------------
+# device-level calls (inputs and outputs are device tensors)
+
+def leapfrog(potential, w0, dt, n_steps, save_stride=1, save=True, n_steps_per_orbit=None,
+             t0=0.0):
+    """Leapfrog on device.  w0: (6, N) float64 device tensor.
+    Returns (w_out (6, n_steps//save_stride + 1, N) or None, w_final (6, N))."""
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    N = w0.shape[1]
+    wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, N)) if save else None
+    wfin = dev.empty((6, N))
+    nsp = None
+    if n_steps_per_orbit is not None:
+        nsp = dev.to_dev(n_steps_per_orbit, dtype=dev.torch().int64)
+        n_steps = int(nsp.max().item()) if N else 0
+    _lib.check(lib.gamd_leapfrog(potential.c_handle, w0.data_ptr(), N, float(t0), float(dt),
+                                 int(n_steps), dev.ptr(nsp), int(save_stride), dev.ptr(wout),
+                                 wfin.data_ptr(), dev.stream_ptr()), "gamd_leapfrog")
+    return wout, wfin
+
+
+def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0, hmax=0.0,
+           t_out=None, order=None, block=0):
+    """Per-orbit adaptive DOP853 on device.  w0 (6, N); t0, t1, h0 per orbit
+    (scalars broadcast).  Returns dict(w_final, status, stats[, w_out])."""
+    t = dev.require_gpu()
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    N = w0.shape[1]
+
+    def per_orbit(v):
+        if v is None:
+            return None
+        if isinstance(v, t.Tensor):
+            return dev.to_dev(v.expand(N) if v.dim() == 0 else v)
+        return dev.to_dev(np.broadcast_to(np.asarray(v, dtype=np.float64), (N,)))
+
+    t0d, t1d, h0d = per_orbit(t0), per_orbit(t1), per_orbit(h0)
+    wfin = dev.empty((6, N))
+    status = dev.empty((N,), dtype=t.int32)
+    stats = dev.empty((N, 4), dtype=t.int64)
+    wout = tout = None
+    if t_out is not None:
+        tout = dev.to_dev(t_out)
+        wout = dev.empty((6, tout.shape[0], N))
+    ordd = dev.to_dev(order, dtype=t.int32) if order is not None else None
+    _lib.check(lib.gamd_dop853(potential.c_handle, w0.data_ptr(), N, t0d.data_ptr(),
+                               t1d.data_ptr(), dev.ptr(h0d), float(rtol), float(atol), int(nmax),
+                               int(nstiff), float(hmax), dev.ptr(tout),
+                               0 if tout is None else tout.shape[0], dev.ptr(wout),
+                               wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
+                               dev.ptr(ordd), int(block), dev.stream_ptr()), "gamd_dop853")
+    out = {"w_final": wfin, "status": status, "stats": stats}
+    if wout is not None:
+        out["w_out"] = wout
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Hamiltonian.integrate_orbit
+
+def _prepare_w0(hamiltonian, w0):
+    from .dynamics import PhaseSpacePosition
+    if isinstance(w0, PhaseSpacePosition):
+        arr = w0.w(hamiltonian.units)
+    else:
+        arr = np.asarray(w0, dtype=np.float64)
+    if arr.shape[0] != hamiltonian.ndim:
+        raise ValueError(f"Invalid initial conditions shape {arr.shape}. Expected shape "
+                         f"({hamiltonian.ndim}, ...) but got shape {arr.shape}.")
+    orig_shape = arr.shape
+    return orig_shape, np.ascontiguousarray(arr.reshape(arr.shape[0], -1))
+
+
+def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=None,
+                          save_all=True, **time_spec):
+    """The dispatch of chamiltonian.pyx:274-380 onto the HIP kernels.
+
+    Leapfrog: dt = t[1]-t[0], len(t)-1 steps (leapfrog.pyx:78, 106).
+    DOP853: every orbit is its own adaptive system from t[0] to t[-1] with
+    h0 = t[1]-t[0] (gala with nbatch=1, dop853.pyx:171, 228-245); with
+    save_all the dense output samples every t (dop853.cpp:584-612).
+    Integrator_kwargs: atol, rtol (default 1e-10), nmax, err_if_fail
+    (default 1, raise on a negative status), nstiff, hmax.
+    """
+    from .dynamics import Orbit
+    Integrator_kwargs = dict(Integrator_kwargs or {})
+    if Integrator is None:
+        Integrator = LeapfrogIntegrator
+    Integrator = get_integrator(Integrator)
+    orig_shape, arr = _prepare_w0(hamiltonian, w0)
+    t = np.ascontiguousarray(parse_time_specification(hamiltonian.units, **time_spec))
+    pot = hamiltonian.potential
+    if Integrator is LeapfrogIntegrator:
+        if len(t) < 2:
+            raise ValueError("need at least two times")
+        dt = t[1] - t[0]
+        wout, wfin = leapfrog(pot, arr, dt, len(t) - 1, save_stride=1, save=bool(save_all),
+                              t0=t[0])
+        w = dev.to_host(wout) if save_all else dev.to_host(wfin)
+    elif Integrator is DOPRI853Integrator:
+        res = dop853(pot, arr, t[0], t[-1], t[1] - t[0],
+                     rtol=Integrator_kwargs.get("rtol", 1e-10),
+                     atol=Integrator_kwargs.get("atol", 1e-10),
+                     nmax=Integrator_kwargs.get("nmax", 0),
+                     nstiff=Integrator_kwargs.get("nstiff", 0),
+                     hmax=Integrator_kwargs.get("dt_max", 0.0),
+                     t_out=t if save_all else None)
+        status = dev.to_host(res["status"])
+        if int(Integrator_kwargs.get("err_if_fail", 1)) == 1 and np.any(status < 0):
+            bad = int(status[status < 0][0])
+            raise RuntimeError(f"Integration failed with code {bad}")
+        w = dev.to_host(res["w_out"]) if save_all else dev.to_host(res["w_final"])
+    else:
+        raise ValueError(f"Cython integration not supported for '{Integrator!r}'")
+    if w.shape[-1] == 1:  # chamiltonian.pyx:355-356
+        w = w[..., 0]
+    if not save_all:
+        w = w[:, None]
+        t = t[-1:]
+    return Orbit.from_w(w=w, units=hamiltonian.units, t=t, hamiltonian=hamiltonian)

@@ -1,0 +1,97 @@
+"""GPU DOP853 (gamd_dop853) against the oracle's restatement of gala's
+dop853 driven with nbatch=1 (one adaptive system per orbit), which the
+oracle matches bit for bit against gala's own compiled dop853.cpp
+(tests/test_oracle_ref.py).
+
+Bars: identical accepted-step counts for all but a tiny fraction of orbits
+(an ulp-level difference can flip one accept/reject decision), final states
+within 1e-8 relative where the step sequences agree, and dense output within
+1e-8 relative (the stated DOP853 tolerance)."""
+import numpy as np
+import pytest
+
+from gala_amd import workloads
+from oracle import oracle as O
+from tests.helpers import mw_oracle, mw_product, state_rel_err, vcirc_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_per_orbit_uneven_parity():
+    from gala_amd.integrate import dop853
+    from gala_amd import _device as dev
+    orc = mw_oracle()
+    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=2048, seed=1)
+    T = T * 0.2  # keep the oracle run short
+    ref, st_ref, stats_ref = O.dop853_orbits(orc, w0, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
+    # each orbit's own sensitivity: the oracle again from w0 moved by one ulp
+    pert = w0.copy()
+    pert[0] = np.nextafter(pert[0], np.inf)
+    alt, _, stats_alt = O.dop853_orbits(orc, pert, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
+    sens = state_rel_err(alt, ref)
+    tol = np.maximum(1e-8, 100.0 * sens)
+    res = dop853(mw_product(), w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
+    w = dev.to_host(res["w_final"])
+    st = dev.to_host(res["status"])
+    stats = dev.to_host(res["stats"])
+    assert np.array_equal(st, st_ref)
+    same = stats[:, 1] == stats_ref[:, 1]
+    own = (stats_alt[:, 1] == stats_ref[:, 1]).mean()
+    print(f"identical accepted-step counts: GPU {same.mean():.4f}, oracle under a one-ulp "
+          f"change {own:.4f}; {np.sum(tol > 1e-8)} orbit(s) gated by their own sensitivity")
+    assert 1.0 - same.mean() <= 3.0 * (1.0 - own) + 0.005
+    err = state_rel_err(w, ref)
+    assert np.all(err <= tol)
+    assert np.median(err) < 1e-10
+
+
+def test_dense_output_parity():
+    from gala_amd.potential import Hamiltonian
+    orc = mw_oracle()
+    rng = np.random.default_rng(5)
+    n = 64
+    w0 = np.vstack([rng.uniform(3, 20, (3, n)), rng.normal(0, 0.12, (3, n))])
+    t = np.linspace(0.0, 600.0, 301)
+    ref, st, _ = O.dop853(orc, w0, t, atol=1e-10, rtol=1e-10, nbatch=1, save_all=True)
+    orbit = Hamiltonian(mw_product()).integrate_orbit(
+        w0, t=t, Integrator="dopri853", Integrator_kwargs=dict(atol=1e-10, rtol=1e-10))
+    w = orbit.w()
+    assert w.shape == (6, 301, n)
+    assert np.max(state_rel_err(w, ref)) < 1e-8
+    # interpolant at the interval ends reproduces the first output exactly
+    assert np.array_equal(w[:, 0], w0)
+
+
+def test_status_codes_and_errors():
+    from gala_amd.integrate import dop853
+    from gala_amd import _device as dev
+    from gala_amd.potential import Hamiltonian
+    pot = mw_product()
+    w0 = np.array([[8.0], [0.0], [0.0], [0.0], [0.2], [0.0]])
+    # nmax exceeded -> -2 (dop853.cpp:340-347)
+    res = dop853(pot, w0, 0.0, 5000.0, 1.0, rtol=1e-12, atol=1e-12, nmax=5)
+    assert dev.to_host(res["status"])[0] == -2
+    st_o = O.dop853_orbits(mw_oracle(), w0, 0.0, 5000.0, 1.0, atol=1e-12, rtol=1e-12, nmax=5)[1]
+    assert st_o[0] == -2
+    with pytest.raises(RuntimeError):
+        Hamiltonian(pot).integrate_orbit(w0[:, 0], dt=1.0, n_steps=5000, Integrator="dop853",
+                                         Integrator_kwargs=dict(nmax=5))
+    # h0 = 0 -> Hairer's initial step guess (dop853.cpp:18-86), still correct
+    res = dop853(pot, w0, 0.0, 500.0, 0.0, rtol=1e-10, atol=1e-10)
+    ref = O.dop853_orbits(mw_oracle(), w0, 0.0, 500.0, 1.0, atol=1e-10, rtol=1e-10)[0]
+    assert state_rel_err(dev.to_host(res["w_final"]), ref)[0] < 1e-8
+
+
+def test_queue_order_invariance():
+    """The work-queue order changes which lane runs an orbit, never its result."""
+    from gala_amd.integrate import dop853
+    from gala_amd import _device as dev
+    orc = mw_oracle()
+    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=5000, seed=9)
+    T = T * 0.05
+    pot = mw_product()
+    a = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
+    order = np.random.default_rng(0).permutation(w0.shape[1]).astype(np.int32)
+    b = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, order=order, block=128)
+    assert np.array_equal(dev.to_host(a["w_final"]), dev.to_host(b["w_final"]))
+    assert np.array_equal(dev.to_host(a["stats"]), dev.to_host(b["stats"]))
