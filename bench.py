@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Benchmark: orbit-steps/s on one MI355X.
+
+Default workload (the one the metric is quoted on): batched uneven DOP853,
+1,048,576 orbits in the Milky-Way-like composite potential, FP64 state,
+rtol 1e-8 / atol 1e-10, per-orbit integration time logU(0.1, 10) Gyr, all
+orbits in one persistent launch (gamd_dop853).  One "step" of this script is
+one full integration of every orbit; orbit-steps are ACCEPTED DOP853 steps
+(function evaluations/s reported beside them).  Inputs are resident in HBM
+before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line with the metric, a roofline object for the dominant
+kernel (achieved algorithmic FP64 TFLOP/s from HIP events on the launch
+stream against the 78.6 TFLOP/s FP64 peak) and a cpu_baseline object (the
+oracle's -Ofast OpenMP restatement of gala's C loops on a bounded sample of
+the same workload, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "orbit-steps/sec @1 MI355X: exact FP64 vs SCF-surrogate at ≤1e-4 dE/E"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline_dop853(w0, T, stride):
+    from oracle import oracle as O
+    orc = O.milky_way_like()
+    sel = np.arange(0, w0.shape[1], stride)
+    ws = np.ascontiguousarray(w0[:, sel])
+    t0 = time.perf_counter()
+    _, st, stats = O.dop853_orbits(orc, ws, 0.0, T[sel], 1.0, atol=1e-10, rtol=1e-8, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": float(stats[:, 1].sum() / dt), "unit": "orbit-steps/s",
+            "cores": int(O.num_threads(fast=True)), "kind": "port",
+            "sample": f"every {stride}th orbit of the same workload ({len(sel)} orbits, "
+                      f"{int(stats[:, 1].sum())} accepted steps, {dt:.1f} s), oracle/gala_oracle.c "
+                      f"built -Ofast with OpenMP"}
+
+
+def bench_dop853(args):
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import roofline, workloads
+    from gala_amd.integrate import dop853
+    from gala_amd.potential import MilkyWayLikePotential
+
+    n = args.n or 1048576
+    pot = MilkyWayLikePotential()
+    log(f"generating {n} orbits")
+    w0, T = workloads.composite_dop853(pot.circular_velocity, n=n, seed=1)
+    order = workloads.cost_order(w0, T, pot.gradient)
+    w0d = dev.to_dev(w0)
+    t0d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    t1d = dev.to_dev(T)
+    h0d = torch.ones(n, dtype=torch.float64, device="cuda")
+    ordd = dev.to_dev(order, dtype=torch.int32)
+    torch.cuda.synchronize()
+
+    def one():
+        return dop853(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, order=ordd,
+                      block=args.block)
+
+    for i in range(args.warmup):
+        one()
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    stream = torch.cuda.current_stream()
+    evs = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = one()
+        e1.record(stream)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    stats = dev.to_host(res["stats"])
+    status = dev.to_host(res["status"])
+    naccpt = int(stats[:, 1].sum())
+    nrejct = int(stats[:, 2].sum())
+    nfcn = int(stats[:, 3].sum())
+    ms_per_step = wall * 1000.0 / args.steps
+    value = naccpt / (ms_per_step / 1000.0)
+    log(f"{args.steps} passes: {ms_per_step:.1f} ms/pass, kernel {kernel_ms:.1f} ms; "
+        f"{naccpt} accepted, {nrejct} rejected, status ok {np.mean(status == 1):.6f}")
+    flops = roofline.dop853_flops(naccpt, nrejct)
+    achieved = flops / (kernel_ms / 1000.0) / 1e12
+    traffic = traffic_from_profiles("dop853_composite")
+    out = {
+        "metric": METRIC, "value": value, "unit": "orbit-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded initial conditions, numpy default_rng(1))",
+        "config": {"workload": "composite_dop853", "n_orbits": n,
+                   "potential": "MW-like composite: Hernquist 1.7e9/0.07 + Hernquist 5e9/1.0 + "
+                                "Miyamoto-Nagai 6.8e10/3.0/0.28 + NFW 5.4e11/15.6",
+                   "rtol": 1e-8, "atol": 1e-10, "T": "logU(0.1, 10) Gyr per orbit",
+                   "r0": "logU(1, 200) kpc", "step": "one integration of every orbit",
+                   "orbit_step": "accepted DOP853 step", "parallelism": "single GPU"},
+        "accepted_steps": naccpt, "rejected_steps": nrejct,
+        "fevals_per_s": nfcn / (ms_per_step / 1000.0),
+        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": achieved,
+                     "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / roofline.PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": "k_dop853<false,false>", "kernel_ms": kernel_ms,
+                     "flops_per_launch": flops},
+    }
+    return out, (w0, T)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="composite_dop853")
+    ap.add_argument("--n", type=int, default=0, help="override the orbit count (testing)")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--cpu-stride", type=int, default=4,
+                    help="CPU baseline sample: every k-th orbit")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU only: this build has no multi-GPU path "
+                 "(--gpus must be 1)")
+    import torch
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    if args.workload != "composite_dop853":
+        sys.exit(f"unknown workload {args.workload}")
+    out, (w0, T) = bench_dop853(args)
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline_dop853(w0, T, args.cpu_stride)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

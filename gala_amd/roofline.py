@@ -1,0 +1,43 @@
+"""Algorithmic work per unit, frozen here and used by bench.py.
+
+Counting convention: add, sub, mul, div, sqrt, log, pow = 1 flop each, a
+fused multiply-add = 2; parameter-only subexpressions (G m, b^2, ...) are
+hoisted and not counted.  Counts follow the reference formulas:
+
+  Hernquist grad   builtin_potentials.cpp:221-233        16
+  Plummer grad     :302-314                              15
+  Miyamoto-Nagai   :1298-1312                            20
+  spherical NFW    :835-850                              22
+  composite (nucleus, bulge, disk, halo)                 74
+  leapfrog update  leapfrog.pyx:35-51                    18  -> 92 per orbit-step
+  DOP853 accepted  dop853.cpp:369-459: 12 evaluations (12 x 74) + stage
+                   combinations 666 + b-combination 102 + error norm 174 +
+                   control 10                           ~1,840
+  DOP853 rejected  11 evaluations, no FSAL              ~1,766
+  SCF Phi + grad   (nmax 10, lmax 6; 308 active terms): 8 T + 20 L + 12 R
+                   + 8 L + 50 with T = 308, L = 28, R = 77 ~4,222
+  MLP forward      2 (4*256 + 3*256^2 + 256*4)         397,312 (BF16 MFMA)
+
+These are ALGORITHMIC counts: transcendental functions (sqrt, divide, log)
+expand to many FP64 instructions on CDNA, so the instruction rate rocprof
+reports is higher than these flops imply.
+"""
+
+FLOP_COMPOSITE_GRAD = 74
+FLOP_LEAPFROG_STEP = 92
+FLOP_DOP853_ACCEPTED = 1840
+FLOP_DOP853_REJECTED = 1766
+FLOP_DOP853_DENSE_EXTRA = 1140
+FLOP_SCF_GRAD = 4222
+FLOP_SCF_LEAPFROG_STEP = 4240
+FLOP_MLP_FORWARD = 2 * (4 * 256 + 3 * 256 * 256 + 256 * 4)
+BYTES_LEAPFROG_SAVE = 48  # one (x, v) state written per saved orbit-step
+
+# MI355X published peaks
+PEAK_FP64_TFLOPS = 78.6      # FP64 vector (and FP64 matrix) dense
+PEAK_BF16_TFLOPS = 2500.0    # BF16 dense MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def dop853_flops(naccpt, nrejct):
+    return naccpt * FLOP_DOP853_ACCEPTED + nrejct * FLOP_DOP853_REJECTED

@@ -25,10 +25,15 @@ def test_per_orbit_uneven_parity():
     T = T * 0.2  # keep the oracle run short
     ref, st_ref, stats_ref = O.dop853_orbits(orc, w0, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
     # each orbit's own sensitivity: the oracle again from w0 moved by one ulp
-    pert = w0.copy()
-    pert[0] = np.nextafter(pert[0], np.inf)
-    alt, _, stats_alt = O.dop853_orbits(orc, pert, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
-    sens = state_rel_err(alt, ref)
+    # in x, in y and in v_z (the largest of the three responses)
+    sens = np.zeros(w0.shape[1])
+    for k, direction in ((0, np.inf), (1, -np.inf), (5, np.inf)):
+        pert = w0.copy()
+        pert[k] = np.nextafter(pert[k], direction)
+        alt, _, stats_k = O.dop853_orbits(orc, pert, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
+        sens = np.maximum(sens, state_rel_err(alt, ref))
+        if k == 0:
+            stats_alt = stats_k
     tol = np.maximum(1e-8, 100.0 * sens)
     res = dop853(mw_product(), w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
     w = dev.to_host(res["w_final"])
@@ -41,7 +46,11 @@ def test_per_orbit_uneven_parity():
           f"change {own:.4f}; {np.sum(tol > 1e-8)} orbit(s) gated by their own sensitivity")
     assert 1.0 - same.mean() <= 3.0 * (1.0 - own) + 0.005
     err = state_rel_err(w, ref)
-    assert np.all(err <= tol)
+    bad = np.nonzero(err > tol)[0]
+    for i in bad[:10]:
+        print(f"orbit {i}: err {err[i]:.3e} tol {tol[i]:.3e} naccpt gpu {stats[i, 1]} "
+              f"oracle {stats_ref[i, 1]} |r0| {np.linalg.norm(w0[:3, i]):.3f} T {T[i]:.1f}")
+    assert len(bad) == 0
     assert np.median(err) < 1e-10
 
 
