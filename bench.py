@@ -128,6 +128,7 @@ def bench_dop853(args):
                    "r0": "logU(1, 200) kpc", "step": "one integration of every orbit",
                    "orbit_step": "accepted DOP853 step", "parallelism": "single GPU"},
         "accepted_steps": naccpt, "rejected_steps": nrejct,
+        "max_attempts_one_orbit": int(stats[:, 0].max()),
         "fevals_per_s": nfcn / (ms_per_step / 1000.0),
         "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": achieved,
                      "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

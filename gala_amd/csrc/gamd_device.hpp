@@ -34,9 +34,72 @@ struct DevPot {
     int32_t n;
     int32_t has_scf;
     int32_t all_unshifted;
-    int32_t pad;
+    int32_t layout;       // LAYOUT_* below: selects a straight-line evaluator
     DevComp c[MAXC];
 };
+
+// Component layouts with a dedicated straight-line evaluator.  With a runtime
+// loop over components each component is its own basic block and the four
+// dependent sqrt/divide/log chains run one after the other; written out
+// straight, the scheduler interleaves them, which is what a lone lane in the
+// tail of a persistent launch needs.
+enum { LAYOUT_GENERIC = 0, LAYOUT_MW = 1 };  // MW: Hernquist, Hernquist, MN, NFW, unshifted
+
+// ---------------------------------------------------------------------------
+// Natural log with a short dependency chain, for finite x > 0.  The library
+// log costs ~470 cycles of latency for one lane on gfx950 (a divide ~76, an
+// FMA ~6), and a lone lane in the tail of a persistent launch is latency-
+// bound.  Method: x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m =
+// 2 atanh(s) = 2 s sum_k s^(2k)/(2k+1), s = (m-1)/(m+1), |s| <= 0.1716, so
+// twelve terms reach 1e-20; the series is summed by Estrin's scheme (depth
+// ~6 FMAs) and e ln2 is added with ln2 split in two (the high part has 21
+// trailing zero bits, so e * ln2_hi is exact).  Error ~1-2 ulp.
+__device__ __forceinline__ double log_fast(double x) {
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.7071067811865476) {
+        m = m + m;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+    const double p01 = fma(z, 0.3333333333333333, 1.0);
+    const double p23 = fma(z, 0.14285714285714285, 0.2);
+    const double p45 = fma(z, 0.09090909090909091, 0.1111111111111111);
+    const double p67 = fma(z, 0.06666666666666667, 0.07692307692307693);
+    const double p89 = fma(z, 0.05263157894736842, 0.058823529411764705);
+    const double pab = fma(z, 0.043478260869565216, 0.047619047619047616);
+    const double q03 = fma(z2, p23, p01);
+    const double q47 = fma(z2, p67, p45);
+    const double q8b = fma(z2, pab, p89);
+    const double P = fma(z8, q8b, fma(z4, q47, q03));
+    const double ed = (double)e;
+    const double lm = (s + s) * P;
+    return fma(ed, 0.6931471803691238, fma(ed, 1.9082149292705877e-10, lm));
+}
+
+// 1/sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64) refined by
+// two Newton steps y <- y + y (1/2 - (x/2) y^2) in FMA form (~1 ulp; a
+// correctly rounded sqrt is ~110 cycles of latency on gfx950, this ~60).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    double r = fma(-hx * y, y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-hx * y, y, 0.5);
+    return fma(y, r, y);
+}
+
+// 1/d for finite nonzero d: v_rcp_f64 refined by two Newton steps (~1 ulp).
+__device__ __forceinline__ double rcp_nr(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    return fma(y, e, y);
+}
 
 // ---------------------------------------------------------------------------
 // analytic components: accumulate the gradient (gala's *_gradient_single
@@ -556,6 +619,82 @@ __device__ __forceinline__ void pot_grad(const DevPot &P, double x, double y, do
             gz += c.R[2] * tx + c.R[5] * ty + c.R[8] * tz;
         }
     }
+}
+
+// The Milky-Way layout (nucleus and bulge Hernquist, Miyamoto-Nagai disk, NFW
+// halo, gala's MilkyWayPotential order): the same formulas as the per-
+// component functions above, one shared |q|, contributions accumulated in
+// table order.
+__device__ __forceinline__ void mw_grad(const DevPot &P, double x, double y, double z,
+                                        double &gx, double &gy, double &gz) {
+    const DevComp &h1 = P.c[0], &h2 = P.c[1], &mn = P.c[2], &nf = P.c[3];
+    const double R2 = x * x + y * y;
+    const double r2 = R2 + z * z;
+    // |q|, 1/|q|^3, sqrt(z^2 + b^2), its inverse and D^(-3/2) from three
+    // reciprocal square roots instead of three correctly rounded roots and
+    // their divides
+    const double ir = rsqrt_nr(r2);
+    const double r = r2 * ir;
+    const double ir3 = ir * ir * ir;
+    const double s2 = z * z + mn.d[2];
+    const double isz = rsqrt_nr(s2);
+    const double sqrtz = s2 * isz;
+    const double zd = mn.d[1] + sqrtz;
+    const double D = R2 + zd * zd;
+    const double iD = rsqrt_nr(D);
+    const double f3 = mn.d[0] * (iD * iD * iD);
+    const double f3z = 1.0 + mn.d[1] * isz;
+    // NFW in the equivalent form G m / r^3 (log(1+u) - u/(1+u)), u = r / r_s
+    // (v_h2 / u^3 / r_s^2 = G m / r^3)
+    const double u = r * nf.d[3];  // d[3] = 1 / r_s
+    const double opu = 1.0 + u;
+    const double lg = log_fast(opu);
+    // The remaining denominators share one reciprocal (batched inversion:
+    // invert the product, peel the factors off with multiplies).
+    const double rc1 = r + h1.d[1];
+    const double rc2 = r + h2.d[1];
+    const double d1 = rc1 * rc1 * r, d2 = rc2 * rc2 * r;
+    const double p12 = d1 * d2;
+    const double p = p12 * opu;
+    double f1, f2, f4;
+    if (p > 1e-280 && p < 1e280) {
+        const double inv = rcp_nr(p);
+        const double i4 = inv * p12, inv2 = inv * opu;
+        const double i1 = inv2 * d2, i2 = inv2 * d1;
+        f1 = h1.d[0] * i1;
+        f2 = h2.d[0] * i2;
+        f4 = nf.d[0] * nf.d[1] * (lg - u * i4) * ir3;
+    } else {  // extreme radii (or r = 0): the per-component forms
+        f1 = h1.d[0] / d1;
+        f2 = h2.d[0] / d2;
+        f4 = nf.d[0] / (u * u * u) / nf.d[2] * (lg - u / opu);
+    }
+    gx = 0.0;
+    gy = 0.0;
+    gz = 0.0;
+    gx += f1 * x;
+    gy += f1 * y;
+    gz += f1 * z;
+    gx += f2 * x;
+    gy += f2 * y;
+    gz += f2 * z;
+    gx += f3 * x;
+    gy += f3 * y;
+    gz += f3 * z * f3z;
+    gx += f4 * x;
+    gy += f4 * y;
+    gz += f4 * z;
+}
+
+// grad Phi with the layout chosen at compile time (LAYOUT_GENERIC reads the
+// table's runtime layout and falls back to the general loop)
+template <bool WITH_SCF, int LAYOUT>
+__device__ __forceinline__ void pot_grad_l(const DevPot &P, double x, double y, double z,
+                                           double &gx, double &gy, double &gz) {
+    if (LAYOUT == LAYOUT_MW)
+        mw_grad(P, x, y, z, gx, gy, gz);
+    else
+        pot_grad<WITH_SCF>(P, x, y, z, gx, gy, gz);
 }
 
 template <bool WITH_SCF>

@@ -50,10 +50,10 @@ struct DopArgs {
 #define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
 
 // f = (p, -grad Phi(q))
-template <bool SCF>
+template <bool SCF, int LAY>
 __device__ __forceinline__ void rhs(const DevPot &P, const double *y, double *f) {
     double gx, gy, gz;
-    pot_grad<SCF>(P, y[0], y[1], y[2], gx, gy, gz);
+    pot_grad_l<SCF, LAY>(P, y[0], y[1], y[2], gx, gy, gz);
     f[0] = y[3];
     f[1] = y[4];
     f[2] = y[5];
@@ -70,7 +70,7 @@ __device__ __forceinline__ double min_d(double a, double b) { return (a < b) ? a
 __device__ __forceinline__ double max_d(double a, double b) { return (a > b) ? a : b; }
 
 // Hairer's initial step guess, dop853.cpp:18-86 (scalar tolerances).
-template <bool SCF>
+template <bool SCF, int LAY>
 __device__ double hinit(const DevPot &P, double x, const double *y, double posneg,
                         const double *f0, double hmax, double rtol, double atol) {
     double dnf = 0.0, dny = 0.0;
@@ -86,7 +86,7 @@ __device__ double hinit(const DevPot &P, double x, const double *y, double posne
     h = sign_d(h, posneg);
     double yy[6], f1[6];
     FOR6 yy[c] = y[c] + h * f0[c];
-    rhs<SCF>(P, yy, f1);
+    rhs<SCF, LAY>(P, yy, f1);
     double der2 = 0.0;
     FOR6 {
         const double sk = atol + rtol * fabs(y[c]);
@@ -95,17 +95,17 @@ __device__ double hinit(const DevPot &P, double x, const double *y, double posne
     }
     der2 = sqrt(der2) / h;
     const double der12 = max_d(fabs(der2), sqrt(dnf));
-    const double h1 = (der12 <= 1.0e-15) ? max_d(1.0e-6, fabs(h) * 1.0e-3) : pow(0.01 / der12, 1.0 / 8.0);
+    const double h1 = (der12 <= 1.0e-15) ? max_d(1.0e-6, fabs(h) * 1.0e-3) : sqrt(sqrt(sqrt(0.01 / der12)));
     h = min_d(100.0 * fabs(h), min_d(h1, hmax));
     return sign_d(h, posneg);
 }
 
-template <bool SCF, bool DENSE>
+template <bool SCF, bool DENSE, int LAY>
 __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a) {
     const DevPot &P = *Pp;
     const int lane = threadIdx.x & 63;
     const double uround = 2.220446049250313e-16;  // np.finfo(float).eps (dop853.pyx:165)
-    const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0, expo1 = 1.0 / 8.0;
+    const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
 
     bool active = false, exhausted = false;
     int64_t orb = 0;
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                     nfcn = 2;
                     oidx = 0;
                     reject = iasti = nonsti = 0;
-                    rhs<SCF>(P, y, k1);
-                    if (h == 0.0) h = hinit<SCF>(P, x, y, posneg, k1, hmax, a.rtol, a.atol);
+                    rhs<SCF, LAY>(P, y, k1);
+                    if (h == 0.0) h = hinit<SCF, LAY>(P, x, y, posneg, k1, hmax, a.rtol, a.atol);
                     active = true;
                 } else {
                     exhausted = true;
@@ -168,37 +168,37 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
             nstep++;
             double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], yy[6];
             FOR6 yy[c] = y[c] + h * T::A[1][0] * k1[c];
-            rhs<SCF>(P, yy, k2);
+            rhs<SCF, LAY>(P, yy, k2);
             FOR6 yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
-            rhs<SCF>(P, yy, k3);
+            rhs<SCF, LAY>(P, yy, k3);
             FOR6 yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
-            rhs<SCF>(P, yy, k4);
+            rhs<SCF, LAY>(P, yy, k4);
             FOR6 yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
-            rhs<SCF>(P, yy, k5);
+            rhs<SCF, LAY>(P, yy, k5);
             FOR6 yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
-            rhs<SCF>(P, yy, k6);
+            rhs<SCF, LAY>(P, yy, k6);
             FOR6 yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
                                      T::A[6][5] * k6[c]);
-            rhs<SCF>(P, yy, k7);
+            rhs<SCF, LAY>(P, yy, k7);
             FOR6 yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
                                      T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
-            rhs<SCF>(P, yy, k8);
+            rhs<SCF, LAY>(P, yy, k8);
             FOR6 yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
                                      T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
-            rhs<SCF>(P, yy, k9);
+            rhs<SCF, LAY>(P, yy, k9);
             FOR6 yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
                                      T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
                                      T::A[9][8] * k9[c]);
-            rhs<SCF>(P, yy, k10);
+            rhs<SCF, LAY>(P, yy, k10);
             FOR6 yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
                                      T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
                                      T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
-            rhs<SCF>(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
+            rhs<SCF, LAY>(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
             const double xph = x + h;
             FOR6 yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
                                      T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
                                      T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
-            rhs<SCF>(P, yy, k3);  // stage 12 reuses k3 (:408)
+            rhs<SCF, LAY>(P, yy, k3);  // stage 12 reuses k3 (:408)
             nfcn += 11;
             FOR6 {
                 k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
@@ -209,19 +209,21 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
             double err = 0.0, err2 = 0.0;
             FOR6 {
                 const double sk = a.atol + a.rtol * max_d(fabs(y[c]), fabs(k5[c]));
+                const double isk = 1.0 / sk;  // one divide serves both estimates
                 double erri = k4[c] - T::BHH[0] * k1[c] - T::BHH[8] * k9[c] - T::BHH[11] * k3[c];
-                double sqr = erri / sk;
+                double sqr = erri * isk;
                 err2 += sqr * sqr;
                 erri = T::E5[0] * k1[c] + T::E5[5] * k6[c] + T::E5[6] * k7[c] + T::E5[7] * k8[c] +
                        T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
-                sqr = erri / sk;
+                sqr = erri * isk;
                 err += sqr * sqr;
             }
             double deno = err + 0.01 * err2;
             if (deno <= 0.0) deno = 1.0;
             err = fabs(h) * err * sqrt(1.0 / (deno * 6.0));
-            // step size factor (:447-452), beta = 0
-            const double fac11 = pow(err, expo1);
+            // step size factor (:447-452), beta = 0: err^(1/8) as three square
+            // roots (~330 cycles of latency where pow's log/exp pair is ~1000)
+            const double fac11 = sqrt(sqrt(sqrt(err)));
             const double fac = max_d(facc2, min_d(facc1, fac11 / safe));
             double hnew = h / fac;
 
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                 // accepted (:454-637)
                 facold = max_d(err, 1.0e-4);
                 naccpt++;
-                rhs<SCF>(P, k5, k4);  // FSAL
+                rhs<SCF, LAY>(P, k5, k4);  // FSAL
                 nfcn++;
                 // stiffness test (:463-486); returns -4 on the first detection
                 if (a.nstiff > 0 && (!(naccpt % a.nstiff) || iasti > 0)) {
@@ -278,17 +280,17 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                                                  T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
                                                  T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
                                                  T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
-                        rhs<SCF>(P, yy, k10);
+                        rhs<SCF, LAY>(P, yy, k10);
                         FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
                                                  T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
                                                  T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
                                                  T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
-                        rhs<SCF>(P, yy, k2);
+                        rhs<SCF, LAY>(P, yy, k2);
                         FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
                                                  T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
                                                  T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
                                                  T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
-                        rhs<SCF>(P, yy, k3);
+                        rhs<SCF, LAY>(P, yy, k3);
                         nfcn += 3;
                         FOR6 {
                             r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
@@ -350,9 +352,9 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
     }
 }
 
-template <bool SCF, bool DENSE>
+template <bool SCF, bool DENSE, int LAY>
 int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
-    auto kern = k_dop853<SCF, DENSE>;
+    auto kern = k_dop853<SCF, DENSE, LAY>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess ||
         per_cu <= 0)
@@ -406,10 +408,15 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     a.queue = queue;
     int rc;
     const bool scf = pot->dev.has_scf;
+    const bool mw = pot->dev.layout == LAYOUT_MW;
     if (w_out)
-        rc = scf ? launch<true, true>(pot->ddev, a, block, s) : launch<false, true>(pot->ddev, a, block, s);
+        rc = scf  ? launch<true, true, LAYOUT_GENERIC>(pot->ddev, a, block, s)
+             : mw ? launch<false, true, LAYOUT_MW>(pot->ddev, a, block, s)
+                  : launch<false, true, LAYOUT_GENERIC>(pot->ddev, a, block, s);
     else
-        rc = scf ? launch<true, false>(pot->ddev, a, block, s) : launch<false, false>(pot->ddev, a, block, s);
+        rc = scf  ? launch<true, false, LAYOUT_GENERIC>(pot->ddev, a, block, s)
+             : mw ? launch<false, false, LAYOUT_MW>(pot->ddev, a, block, s)
+                  : launch<false, false, LAYOUT_GENERIC>(pot->ddev, a, block, s);
     hipError_t fe = hipFreeAsync(queue, s);
     if (rc == GAMD_OK && fe != hipSuccess) rc = GAMD_ERR_HIP;
     return rc;

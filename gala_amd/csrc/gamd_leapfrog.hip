@@ -17,7 +17,7 @@ using namespace gamd;
 
 namespace {
 
-template <bool SCF, bool SAVE>
+template <bool SCF, bool SAVE, int LAY>
 __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp, const double *__restrict__ w0,
                                                   int64_t N, double dt, int64_t n_steps,
                                                   const int64_t *__restrict__ nsteps_i,
@@ -40,14 +40,14 @@ __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp,
         wout[5 * s + i] = vz;
     }
     double gx, gy, gz;
-    pot_grad<SCF>(P, x, y, z, gx, gy, gz);
+    pot_grad_l<SCF, LAY>(P, x, y, z, gx, gy, gz);
     double hx = vx - gx * dt / 2.0, hy = vy - gy * dt / 2.0, hz = vz - gz * dt / 2.0;
     int64_t next_save = stride;
     for (int64_t j = 1; j <= ns; j++) {
         x = x + hx * dt;
         y = y + hy * dt;
         z = z + hz * dt;
-        pot_grad<SCF>(P, x, y, z, gx, gy, gz);
+        pot_grad_l<SCF, LAY>(P, x, y, z, gx, gy, gz);
         vx = hx - gx * dt / 2.0;
         vy = hy - gy * dt / 2.0;
         vz = hz - gz * dt / 2.0;
@@ -90,20 +90,14 @@ extern "C" int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t
     const unsigned g = grid_for(N, LF_BLOCK);
     hipStream_t s = as_stream(stream);
     const bool scf = pot->dev.has_scf;
-    if (w_out) {
-        if (scf)
-            hipLaunchKernelGGL((k_leapfrog<true, true>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
-                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
-        else
-            hipLaunchKernelGGL((k_leapfrog<false, true>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
-                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
-    } else {
-        if (scf)
-            hipLaunchKernelGGL((k_leapfrog<true, false>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
-                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
-        else
-            hipLaunchKernelGGL((k_leapfrog<false, false>), dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0,
-                               (int64_t)N, dt, n_steps, n_steps_per_orbit, save_stride, nsave, w_out, w_final);
-    }
+    const bool mw = pot->dev.layout == LAYOUT_MW;
+    auto kern = w_out ? (scf  ? k_leapfrog<true, true, LAYOUT_GENERIC>
+                         : mw ? k_leapfrog<false, true, LAYOUT_MW>
+                              : k_leapfrog<false, true, LAYOUT_GENERIC>)
+                      : (scf  ? k_leapfrog<true, false, LAYOUT_GENERIC>
+                         : mw ? k_leapfrog<false, false, LAYOUT_MW>
+                              : k_leapfrog<false, false, LAYOUT_GENERIC>);
+    hipLaunchKernelGGL(kern, dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0, (int64_t)N, dt, n_steps,
+                       n_steps_per_orbit, save_stride, nsave, w_out, w_final);
     return launch_status();
 }

@@ -34,6 +34,7 @@ void fold_component(DevComp &c) {
         c.d[0] = p[0] * p[1] / p[2];
         c.d[1] = p[2];
         c.d[2] = p[2] * p[2];
+        c.d[3] = 1.0 / p[2];
         break;
     default:
         break;
@@ -195,6 +196,11 @@ int gamd_potential_create(const gamd_component *components, int ncomponents,
             pot->dev.has_scf = 1;
         }
     }
+    pot->dev.layout = LAYOUT_GENERIC;
+    if (ncomponents == 4 && pot->dev.all_unshifted && pot->dev.c[0].type == GAMD_HERNQUIST &&
+        pot->dev.c[1].type == GAMD_HERNQUIST && pot->dev.c[2].type == GAMD_MIYAMOTONAGAI &&
+        pot->dev.c[3].type == GAMD_NFW)
+        pot->dev.layout = LAYOUT_MW;
     // Kernels read the table through a restrict-qualified pointer: with a
     // wave-uniform component index these become scalar loads, where a by-value
     // kernel argument indexed by a runtime component number would be copied to

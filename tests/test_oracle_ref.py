@@ -1,0 +1,62 @@
+"""The oracle against gala's own compiled C++ (oracle/_ref: cpotential.cpp,
+chamiltonian.cpp, cframe.cpp, builtin_frames.cpp, dop853.cpp built from the
+reference tree, see oracle/ref_driver.cpp).  Both are built with IEEE
+arithmetic (-O2 -ffp-contract=off), so agreement is bit for bit."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import rotation
+
+pytestmark = pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref not built")
+
+
+def composites():
+    R1, R2 = rotation(1), rotation(2)
+    return {
+        "mw": O.milky_way_like(),
+        "shifted": [O.hernquist(3e10, 2.0), O.plummer(1e9, 0.5, q0=[1.0, -2.0, 0.5], R=R1),
+                    O.miyamotonagai(5e10, 2.5, 0.3, R=R2), O.nfw(1e12, 20.0, q0=[0.1, 0.2, 0.3])],
+        "plummer": [O.plummer(1e5, 0.01)],
+    }
+
+
+@pytest.mark.parametrize("name", ["mw", "shifted", "plummer"])
+def test_evaluation_bitwise(name):
+    comps = composites()[name]
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-30, 30, (3, 300))
+    for f in (O.value, O.gradient, O.density, O.d2_dr2):
+        a, b = f(comps, q), f(comps, q, ref=True)
+        assert np.array_equal(a, b), f.__name__
+    if name != "shifted":  # gala does not rotate Hessians back (cpotential.cpp:309-311)
+        assert np.array_equal(O.hessian(comps, q), O.hessian(comps, q, ref=True))
+
+
+@pytest.mark.parametrize("nbatch,save_all", [(1, True), (1, False), (3, True), (100, False)])
+def test_dop853_bitwise(nbatch, save_all):
+    comps = O.milky_way_like()
+    rng = np.random.default_rng(nbatch)
+    n = 9
+    w0 = np.vstack([rng.uniform(2, 25, (3, n)), rng.normal(0, 0.15, (3, n))])
+    t = np.linspace(0.0, 800.0, 201)
+    a, st_a, _ = O.dop853(comps, w0, t, atol=1e-10, rtol=1e-9, nbatch=nbatch, save_all=save_all)
+    b, st_b = O.dop853(comps, w0, t, atol=1e-10, rtol=1e-9, nbatch=nbatch, save_all=save_all,
+                       ref=True)
+    assert np.array_equal(st_a, st_b)
+    assert np.array_equal(a, b)
+
+
+def test_dop853_backward_and_status_codes():
+    comps = O.milky_way_like()
+    w0 = np.array([[8.0], [0.1], [0.2], [0.01], [0.2], [0.02]])
+    t = np.linspace(0.0, -500.0, 51)
+    a, st_a, _ = O.dop853(comps, w0, t, nbatch=1)
+    b, st_b = O.dop853(comps, w0, t, nbatch=1, ref=True)
+    assert np.array_equal(a, b) and st_a[0] == st_b[0] == 1
+    # nmax exceeded (dop853.cpp:340-347)
+    a, st_a, _ = O.dop853(comps, w0, t, nbatch=1, nmax=3, save_all=False, atol=1e-13, rtol=1e-13)
+    b, st_b = O.dop853(comps, w0, t, nbatch=1, nmax=3, save_all=False, atol=1e-13, rtol=1e-13,
+                       ref=True)
+    assert st_a[0] == st_b[0] == -2
+    assert np.array_equal(a, b)
