@@ -29,8 +29,15 @@ class Component(C.Structure):
                 ("R", C.c_double * 9), ("S", C.c_void_p), ("T", C.c_void_p)]
 
 
+class MLPNorm(C.Structure):
+    """gamd_mlp_norm (include/gala_amd.h)."""
+    _fields_ = [("inv_L", C.c_double), ("mu", C.c_double), ("inv_sigma", C.c_double),
+                ("GM", C.c_double), ("r_s", C.c_double), ("pad_r", C.c_double)]
+
+
 # every symbol include/gala_amd.h declares, with its signature
 _vp, _sz, _d, _i64, _i32, _int = C.c_void_p, C.c_size_t, C.c_double, C.c_int64, C.c_int32, C.c_int
+_f, _nm = C.c_float, C.POINTER(MLPNorm)
 SIGNATURES = {
     "gamd_version": (C.c_char_p, []),
     "gamd_potential_create": (_int, [C.POINTER(Component), _int, C.POINTER(_vp)]),
@@ -46,6 +53,16 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _vp, _int, _vp]),
     "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _d, _vp,
                                         _vp]),
+    "gamd_mlp_param_count": (_i64, []),
+    "gamd_mlp_bf16_count": (_i64, []),
+    "gamd_mlp_workspace_bytes": (_i64, [_i64]),
+    "gamd_mlp_sync_bf16": (_int, [_vp, _vp, _vp]),
+    "gamd_mlp_forward": (_int, [_vp, _vp, _nm, _vp, _sz, _vp, _vp, _vp]),
+    "gamd_mlp_make_targets": (_int, [_nm, _vp, _vp, _vp, _sz, _vp, _vp, _vp]),
+    "gamd_mlp_gradients": (_int, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz, _vp]),
+    "gamd_mlp_adam": (_int, [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _vp, _vp]),
+    "gamd_surrogate_leapfrog_step": (_int, [_vp, _vp, _nm, _vp, _vp, _vp, _vp, _vp, _sz, _d,
+                                            _int, _i32, _d, _vp]),
 }
 
 _LIB = None

@@ -43,7 +43,9 @@ struct DevPot {
 // dependent sqrt/divide/log chains run one after the other; written out
 // straight, the scheduler interleaves them, which is what a lone lane in the
 // tail of a persistent launch needs.
-enum { LAYOUT_GENERIC = 0, LAYOUT_MW = 1 };  // MW: Hernquist, Hernquist, MN, NFW, unshifted
+// MW: Hernquist, Hernquist, MN, NFW, unshifted.  SCF106: one unshifted SCF
+// component with nmax = 10, lmax = 6 (the expansion inlined and unrolled).
+enum { LAYOUT_GENERIC = 0, LAYOUT_MW = 1, LAYOUT_SCF106 = 2 };
 
 // ---------------------------------------------------------------------------
 // Natural log with a short dependency chain, for finite x > 0.  The library
@@ -361,6 +363,7 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
         // Gegenbauer recurrences in n
         double c0 = 1.0, c1 = 0.0;  // C_n^a, C_{n-1}^a
         double e0 = 0.0, e1 = 0.0;  // C_{n-1}^{a+1}, C_{n-2}^{a+1}
+#pragma unroll 1
         for (int n = 0; n <= nmax; n++) {
             double Cn, En;  // C_n^a(xi), C_{n-1}^{a+1}(xi)
             if (n == 0) {
@@ -691,10 +694,18 @@ __device__ __forceinline__ void mw_grad(const DevPot &P, double x, double y, dou
 template <bool WITH_SCF, int LAYOUT>
 __device__ __forceinline__ void pot_grad_l(const DevPot &P, double x, double y, double z,
                                            double &gx, double &gy, double &gz) {
-    if (LAYOUT == LAYOUT_MW)
+    if (LAYOUT == LAYOUT_MW) {
         mw_grad(P, x, y, z, gx, gy, gz);
-    else
+    } else if (LAYOUT == LAYOUT_SCF106) {
+        const DevComp &c = P.c[0];
+        double phi;
+        gx = 0.0;
+        gy = 0.0;
+        gz = 0.0;
+        scf_eval_t<10, 6>(c.coef, c.p[0], c.p[1], c.p[2], 10, 6, x, y, z, true, phi, gx, gy, gz);
+    } else {
         pot_grad<WITH_SCF>(P, x, y, z, gx, gy, gz);
+    }
 }
 
 template <bool WITH_SCF>

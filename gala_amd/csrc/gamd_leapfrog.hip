@@ -91,12 +91,15 @@ extern "C" int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t
     hipStream_t s = as_stream(stream);
     const bool scf = pot->dev.has_scf;
     const bool mw = pot->dev.layout == LAYOUT_MW;
-    auto kern = w_out ? (scf  ? k_leapfrog<true, true, LAYOUT_GENERIC>
-                         : mw ? k_leapfrog<false, true, LAYOUT_MW>
-                              : k_leapfrog<false, true, LAYOUT_GENERIC>)
-                      : (scf  ? k_leapfrog<true, false, LAYOUT_GENERIC>
-                         : mw ? k_leapfrog<false, false, LAYOUT_MW>
-                              : k_leapfrog<false, false, LAYOUT_GENERIC>);
+    const bool scf106 = pot->dev.layout == LAYOUT_SCF106;
+    auto kern = w_out ? (scf106 ? k_leapfrog<true, true, LAYOUT_SCF106>
+                         : scf  ? k_leapfrog<true, true, LAYOUT_GENERIC>
+                         : mw   ? k_leapfrog<false, true, LAYOUT_MW>
+                                : k_leapfrog<false, true, LAYOUT_GENERIC>)
+                      : (scf106 ? k_leapfrog<true, false, LAYOUT_SCF106>
+                         : scf  ? k_leapfrog<true, false, LAYOUT_GENERIC>
+                         : mw   ? k_leapfrog<false, false, LAYOUT_MW>
+                                : k_leapfrog<false, false, LAYOUT_GENERIC>);
     hipLaunchKernelGGL(kern, dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0, (int64_t)N, dt, n_steps,
                        n_steps_per_orbit, save_stride, nsave, w_out, w_final);
     return launch_status();

@@ -1,0 +1,704 @@
+// gamd_mlp.hip -- the learned SCF-force surrogate: a 4 -> 256 x 4 (SiLU) -> 4
+// MLP in BF16 on the matrix cores (FP32 accumulation, FP32 master weights),
+// its training kernels (forward, backward, Adam) and the surrogate leapfrog
+// with a per-orbit energy monitor and exact-SCF fallback.  No counterpart in
+// gala: the exact force it stands in for is the SCF expansion of
+// potential/scf/src/bfe.cpp:114-189.
+//
+// Network, per point q (FP64 in, FP64 out):
+//   features  f = (x/L, y/L, z/L, (ln r - mu)/sigma)
+//   hidden    h1 = silu(f W1 + b1), h_l = silu(h_{l-1} W_l + b_l), l = 2..4
+//   output    o = h4 W5 + b5   (4 values)
+//   physics   Phi  = Phi_H(r) (1 + o0),         Phi_H = -G M / (r + r_s)
+//             grad = g_H(r) (r_hat + (o1, o2, o3)), g_H = G M / (r + r_s)^2
+// so the network learns the departure from the expansion's monopole (the
+// S_000 Hernquist term) in units of the local monopole scale.
+//
+// Matrix-core mapping (gfx950, v_mfma_f32_16x16x32_bf16; operand layout
+// A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8);
+// D[i][j] in lane j + 16 (i/4), element i%4):
+//   * a workgroup (4 waves) owns a tile of 128 points; the activations of a
+//     layer live in LDS (128 x 256 BF16, rows padded to 264 to keep the
+//     16-byte fragment reads conflict-free), double buffered;
+//   * wave w computes output columns [64w, 64w+64) of all 128 rows: 8 x 4
+//     MFMA tiles, 128 FP32 accumulators per lane, 32 independent MFMAs per
+//     K step;
+//   * hidden-layer weights are pre-swizzled into B-fragment order, so each
+//     wave reads 1 KB contiguous per (N tile, K step) from L2.
+#include <math.h>
+#include <string.h>
+
+#include "gamd_host.hpp"
+
+using namespace gamd;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int H = 256;        // hidden width
+constexpr int TILE = 128;     // points per workgroup
+constexpr int LDSR = 264;     // padded LDS row (BF16 elements)
+// flat FP32 parameter layout
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 4 * H;
+constexpr int OFF_W2 = OFF_B1 + H, OFF_B2 = OFF_W2 + H * H;
+constexpr int OFF_W3 = OFF_B2 + H, OFF_B3 = OFF_W3 + H * H;
+constexpr int OFF_W4 = OFF_B3 + H, OFF_B4 = OFF_W4 + H * H;
+constexpr int OFF_W5 = OFF_B4 + H, OFF_B5 = OFF_W5 + H * 4;
+constexpr int NPARAM = OFF_B5 + 4;  // 199,684
+// BF16 copies: swizzled W2..W4, then row-major W2..W4
+constexpr int NBF = 6 * H * H;
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ __bf16 tobf(float f) { return (__bf16)f; }
+__device__ __forceinline__ float frombf(__bf16 b) { return (float)b; }
+
+// ---------------------------------------------------------------------------
+// features / physics scaling
+
+__device__ __forceinline__ void features(const gamd_mlp_norm &nm, double x, double y, double z,
+                                         float *f, double &r) {
+    r = sqrt(x * x + y * y + z * z);
+    f[0] = (float)(x * nm.inv_L);
+    f[1] = (float)(y * nm.inv_L);
+    f[2] = (float)(z * nm.inv_L);
+    f[3] = (float)((log(r) - nm.mu) * nm.inv_sigma);
+}
+
+// ---------------------------------------------------------------------------
+// fused forward of one 128-point tile.  feat: LDS [TILE][4]; on return the
+// raw outputs are in LDS out[TILE][4].  With TRAIN, pre-activations and
+// activations of the four hidden layers go to global (BF16, [4][M][256]).
+template <bool TRAIN>
+__device__ void tile_forward(const float *__restrict__ theta, const __bf16 *__restrict__ wsw,
+                             const float (*feat)[4], __bf16 *bufA, __bf16 *bufB, float (*out)[4],
+                             int64_t row0, int64_t M, __bf16 *__restrict__ Zs,
+                             __bf16 *__restrict__ Hs) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // layer 1: 4 -> 256 in FP32; thread t owns unit j = t
+    {
+        const int j = t;
+        const float w0 = theta[OFF_W1 + 0 * H + j], w1 = theta[OFF_W1 + 1 * H + j];
+        const float w2 = theta[OFF_W1 + 2 * H + j], w3 = theta[OFF_W1 + 3 * H + j];
+        const float bj = theta[OFF_B1 + j];
+#pragma unroll 4
+        for (int p = 0; p < TILE; p++) {
+            const float z = bj + feat[p][0] * w0 + feat[p][1] * w1 + feat[p][2] * w2 + feat[p][3] * w3;
+            const float h = silu(z);
+            bufA[p * LDSR + j] = tobf(h);
+            if (TRAIN && row0 + p < M) {
+                Zs[(0 * M + row0 + p) * H + j] = tobf(z);
+                Hs[(0 * M + row0 + p) * H + j] = tobf(h);
+            }
+        }
+    }
+    __syncthreads();
+    // layers 2..4 on the matrix cores
+    __bf16 *in = bufA, *outb = bufB;
+#pragma unroll 1
+    for (int l = 0; l < 3; l++) {
+        const float *bias = theta + (l == 0 ? OFF_B2 : l == 1 ? OFF_B3 : OFF_B4);
+        const __bf16 *W = wsw + (size_t)l * H * H;
+        // the wave's 64 columns in two passes of 32: 64 FP32 accumulators
+        // per lane instead of 128, the A fragments re-read from LDS
+#pragma unroll 1
+        for (int half = 0; half < 2; half++) {
+            f32x4 acc[8][2];
+#pragma unroll
+            for (int mi = 0; mi < 8; mi++)
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+            for (int ks = 0; ks < 8; ks++) {
+                bf16x8 b[2];
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++)
+                    b[ni] = *reinterpret_cast<const bf16x8 *>(
+                        W + (((4 * w + 2 * half + ni) * 8 + ks) * 64 + lane) * 8);
+#pragma unroll
+                for (int mi = 0; mi < 8; mi++) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+                        in + (16 * mi + (lane & 15)) * LDSR + 32 * ks + 8 * (lane >> 4));
+#pragma unroll
+                    for (int ni = 0; ni < 2; ni++)
+                        acc[mi][ni] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ni], acc[mi][ni], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int mi = 0; mi < 8; mi++)
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++) {
+                    const int col = 64 * w + 16 * (2 * half + ni) + (lane & 15);
+                    const float bc = bias[col];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int row = 16 * mi + 4 * (lane >> 4) + e;
+                        const float z = acc[mi][ni][e] + bc;
+                        const float h = silu(z);
+                        outb[row * LDSR + col] = tobf(h);
+                        if (TRAIN && row0 + row < M) {
+                            Zs[((size_t)(l + 1) * M + row0 + row) * H + col] = tobf(z);
+                            Hs[((size_t)(l + 1) * M + row0 + row) * H + col] = tobf(h);
+                        }
+                    }
+                }
+        }
+        __syncthreads();
+        __bf16 *tmp = in;
+        in = outb;
+        outb = tmp;
+    }
+    // layer 5: 256 -> 4 in FP32 from the BF16 activations; thread t: point
+    // t/2, outputs 2(t%2), 2(t%2)+1
+    {
+        const int p = t >> 1, o0 = 2 * (t & 1);
+        float s0 = theta[OFF_B5 + o0], s1 = theta[OFF_B5 + o0 + 1];
+        const __bf16 *hrow = in + p * LDSR;
+#pragma unroll 8
+        for (int j = 0; j < H; j++) {
+            const float hj = frombf(hrow[j]);
+            s0 += hj * theta[OFF_W5 + j * 4 + o0];
+            s1 += hj * theta[OFF_W5 + j * 4 + o0 + 1];
+        }
+        out[p][o0] = s0;
+        out[p][o0 + 1] = s1;
+    }
+    __syncthreads();
+}
+
+constexpr int LDS_BYTES = 2 * TILE * LDSR * 2;
+
+// ---------------------------------------------------------------------------
+// inference: q (3,M) FP64 -> Phi (M), grad (3,M)
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ theta,
+                                                     const __bf16 *__restrict__ wsw,
+                                                     gamd_mlp_norm nm,
+                                                     const double *__restrict__ q, int64_t M,
+                                                     double *__restrict__ phi,
+                                                     double *__restrict__ grad) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *bufB = bufA + TILE * LDSR;
+    __shared__ float feat[TILE][4];
+    __shared__ float out[TILE][4];
+    const int t = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * TILE;
+    double x = 0, y = 0, z = 0, r = 1;
+    if (t < TILE) {
+        const int64_t i = row0 + t;
+        if (i < M) {
+            x = q[i];
+            y = q[M + i];
+            z = q[2 * M + i];
+        } else {
+            x = nm.pad_r;  // a harmless point for the padding rows
+        }
+        float f[4];
+        features(nm, x, y, z, f, r);
+        feat[t][0] = f[0];
+        feat[t][1] = f[1];
+        feat[t][2] = f[2];
+        feat[t][3] = f[3];
+    }
+    __syncthreads();
+    tile_forward<false>(theta, wsw, feat, bufA, bufB, out, row0, M, nullptr, nullptr);
+    if (t < TILE) {
+        const int64_t i = row0 + t;
+        if (i < M) {
+            const double rs = r + nm.r_s;
+            const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
+            phi[i] = phiH * (1.0 + (double)out[t][0]);
+            grad[i] = gH * (x / r + (double)out[t][1]);
+            grad[M + i] = gH * (y / r + (double)out[t][2]);
+            grad[2 * M + i] = gH * (z / r + (double)out[t][3]);
+        }
+    }
+}
+
+// training forward: feat (M,4) FP32 -> Zs, Hs (BF16 [4][M][256]), Y (M,4)
+__global__ __launch_bounds__(256) void k_mlp_forward_train(const float *__restrict__ theta,
+                                                           const __bf16 *__restrict__ wsw,
+                                                           const float *__restrict__ featg,
+                                                           int64_t M, __bf16 *__restrict__ Zs,
+                                                           __bf16 *__restrict__ Hs,
+                                                           float *__restrict__ Y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *bufB = bufA + TILE * LDSR;
+    __shared__ float feat[TILE][4];
+    __shared__ float out[TILE][4];
+    const int t = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * TILE;
+    if (t < TILE) {
+        const int64_t i = row0 + t;
+#pragma unroll
+        for (int c = 0; c < 4; c++) feat[t][c] = i < M ? featg[i * 4 + c] : 0.0f;
+    }
+    __syncthreads();
+    tile_forward<true>(theta, wsw, feat, bufA, bufB, out, row0, M, Zs, Hs);
+    if (t < TILE && row0 + t < M)
+#pragma unroll
+        for (int c = 0; c < 4; c++) Y[(row0 + t) * 4 + c] = out[t][c];
+}
+
+// ---------------------------------------------------------------------------
+// generic BF16 GEMM on the matrix cores for the backward pass:
+//   C[m][n] = sum_k A[m][k] B[k][n], A[m][k] = A[m*sam + k*sak], B[k][n] = B[k*sbk + n*sbn]
+// EPI 0: C (FP32, ldc) = acc.  EPI 1: out (BF16, ldc) = acc * silu'(Zprev[m][n]).
+// Tile 64 x 64 per workgroup (4 waves of 32 x 32); M, N multiples of 64, K of 32.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const __bf16 *__restrict__ A,
+                                              int64_t sam, int64_t sak,
+                                              const __bf16 *__restrict__ B, int64_t sbk,
+                                              int64_t sbn, float *__restrict__ C,
+                                              __bf16 *__restrict__ Cb, int64_t ldc,
+                                              const __bf16 *__restrict__ Zprev) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * 64 + 32 * (w >> 1);
+    const int n0 = blockIdx.y * 64 + 32 * (w & 1);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int li = lane & 15, lk = 8 * (lane >> 4);
+    // split-K over gridDim.z (each slice a multiple of 32; EPI 0 then adds)
+    const int kslice = K / (int)gridDim.z;
+    const int k_lo = (int)blockIdx.z * kslice, k_hi = k_lo + kslice;
+    for (int k0 = k_lo; k0 < k_hi; k0 += 32) {
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int64_t row = m0 + 16 * i + li;
+#pragma unroll
+            for (int e = 0; e < 8; e++) a[i][e] = A[row * sam + (int64_t)(k0 + lk + e) * sak];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int64_t col = n0 + 16 * j + li;
+#pragma unroll
+            for (int e = 0; e < 8; e++) b[j][e] = B[(int64_t)(k0 + lk + e) * sbk + col * sbn];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t row = m0 + 16 * i + 4 * (lane >> 4) + e;
+                const int64_t col = n0 + 16 * j + li;
+                if (EPI == 0) {
+                    if (gridDim.z > 1)
+                        atomicAdd(&C[row * ldc + col], acc[i][j][e]);
+                    else
+                        C[row * ldc + col] = acc[i][j][e];
+                } else
+                    Cb[row * ldc + col] = tobf(acc[i][j][e] * dsilu(frombf(Zprev[row * ldc + col])));
+            }
+}
+
+// loss and dY = 2 (Y - T) / M; per-block partial loss sums
+__global__ void k_loss(const float *__restrict__ Y, const float *__restrict__ T, int64_t M,
+                       float *__restrict__ dY, float *__restrict__ partial) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const int64_t n = M * 4;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float d = Y[i] - T[i];
+        dY[i] = 2.0f * d / (float)M;
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// output layer backward: gW5[j][o] = sum_m H4[m][j] dY[m][o]; gb5[o] = sum_m dY[m][o];
+// dZ4[m][j] = (sum_o dY[m][o] W5[j][o]) silu'(Z4[m][j]).  One thread per (m, j)
+// for dZ4; the reductions over m run per column block with atomics.
+__global__ void k_out_backward_dz(const float *__restrict__ theta, const float *__restrict__ dY,
+                                  const __bf16 *__restrict__ Z4, int64_t M,
+                                  __bf16 *__restrict__ dZ4) {
+    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= M * H) return;
+    const int64_t m = idx / H;
+    const int j = (int)(idx % H);
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 4; o++) s += dY[m * 4 + o] * theta[OFF_W5 + j * 4 + o];
+    dZ4[idx] = tobf(s * dsilu(frombf(Z4[idx])));
+}
+
+// column sums / small reductions: out[j*ncol_out + o] += sum_m X[m][j] Yv[m][o]
+// with X BF16 (M x nx, row stride nx) and Yv FP32 (M x ny).  Grid: (nx, chunks).
+__global__ void k_outer_reduce(const __bf16 *__restrict__ X, int nx, const float *__restrict__ Yv,
+                               int ny, int64_t M, float *__restrict__ out) {
+    const int j = blockIdx.x;
+    const int64_t chunk = (M + gridDim.y - 1) / gridDim.y;
+    const int64_t m_lo = blockIdx.y * chunk;
+    const int64_t m_hi = m_lo + chunk < M ? m_lo + chunk : M;
+    __shared__ float red[256][4];
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t m = m_lo + threadIdx.x; m < m_hi; m += blockDim.x) {
+        const float xv = X ? frombf(X[m * nx + j]) : 1.0f;
+        for (int o = 0; o < ny; o++) s[o] += xv * Yv[m * ny + o];
+    }
+    for (int o = 0; o < 4; o++) red[threadIdx.x][o] = s[o];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int o = 0; o < 4; o++) red[threadIdx.x][o] += red[threadIdx.x + st][o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int o = 0; o < ny; o++) atomicAdd(&out[j * ny + o], red[0][o]);
+}
+
+// bias gradients: gb[j] += sum_m dZ[m][j] (dZ BF16, M x 256).  Grid (256/64... ) simple.
+__global__ void k_colsum(const __bf16 *__restrict__ dZ, int64_t M, float *__restrict__ gb) {
+    const int j = threadIdx.x;  // 256 threads, one column each
+    const int64_t chunk = (M + gridDim.x - 1) / gridDim.x;
+    const int64_t m_lo = blockIdx.x * chunk;
+    const int64_t m_hi = m_lo + chunk < M ? m_lo + chunk : M;
+    float s = 0.f;
+    for (int64_t m = m_lo; m < m_hi; m++) s += frombf(dZ[m * H + j]);
+    atomicAdd(&gb[j], s);
+}
+
+// first-layer weight gradient: gW1[c][j] += sum_m feat[m][c] dZ1[m][j]
+__global__ void k_w1_grad(const float *__restrict__ feat, const __bf16 *__restrict__ dZ1,
+                          int64_t M, float *__restrict__ gW1) {
+    const int j = threadIdx.x;
+    const int64_t chunk = (M + gridDim.x - 1) / gridDim.x;
+    const int64_t m_lo = blockIdx.x * chunk;
+    const int64_t m_hi = m_lo + chunk < M ? m_lo + chunk : M;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t m = m_lo; m < m_hi; m++) {
+        const float d = frombf(dZ1[m * H + j]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) s[c] += feat[m * 4 + c] * d;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) atomicAdd(&gW1[c * H + j], s[c]);
+}
+
+// BF16 copies of W2..W4: swizzled B-fragment order and row-major
+__global__ void k_sync_bf16(const float *__restrict__ theta, __bf16 *__restrict__ wbf) {
+    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= 3 * H * H) return;
+    const int l = (int)(idx / (H * H));
+    const int rem = (int)(idx % (H * H));
+    const int k = rem / H, n = rem % H;  // W[k][n], row-major [in][out]
+    const float v = theta[(l == 0 ? OFF_W2 : l == 1 ? OFF_W3 : OFF_W4) + rem];
+    // fragment order: ((jt * 8 + ks) * 64 + lane) * 8 + e with
+    // n = 16 jt + lane % 16, k = 32 ks + 8 (lane / 16) + e
+    const int jt = n >> 4, ks = k >> 5, lane = (n & 15) + 16 * ((k & 31) >> 3), e = k & 7;
+    wbf[(size_t)l * H * H + (((size_t)jt * 8 + ks) * 64 + lane) * 8 + e] = tobf(v);
+    wbf[(size_t)(3 + l) * H * H + rem] = tobf(v);
+}
+
+// Adam (FP32 masters)
+__global__ void k_adam(float *__restrict__ theta, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, int64_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    theta[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// training data: positions (3,M) FP64 with exact Phi and grad -> features
+// (M,4) and targets (M,4) in the scaled output units
+__global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
+                               const double *__restrict__ phi, const double *__restrict__ grad,
+                               int64_t M, float *__restrict__ feat, float *__restrict__ target) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const double x = q[i], y = q[M + i], z = q[2 * M + i];
+    float f[4];
+    double r;
+    features(nm, x, y, z, f, r);
+#pragma unroll
+    for (int c = 0; c < 4; c++) feat[i * 4 + c] = f[c];
+    const double rs = r + nm.r_s;
+    const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
+    target[i * 4 + 0] = (float)(phi[i] / phiH - 1.0);
+    target[i * 4 + 1] = (float)(grad[i] / gH - x / r);
+    target[i * 4 + 2] = (float)(grad[M + i] / gH - y / r);
+    target[i * 4 + 3] = (float)(grad[2 * M + i] / gH - z / r);
+}
+
+// ---------------------------------------------------------------------------
+// surrogate leapfrog, one launch per step.  Orbit i: w (6,N) holds x and the
+// half-step velocity between launches (the synchronised velocity after the
+// last step); E0[i] the initial energy; fb[i] the step at which the orbit fell
+// back to exact SCF (-1 while on the surrogate).  mode 0: initial half kick;
+// mode 1: drift + kick.  The monitored energy is the surrogate's own
+// Phi + |v|^2/2 against E0; once |E - E0| > tol |E0| the orbit switches to
+// the exact SCF force for the rest of the run (and for the remainder of this
+// step).
+__global__ __launch_bounds__(256) void k_surrogate_step(
+    const float *__restrict__ theta, const __bf16 *__restrict__ wsw, gamd_mlp_norm nm,
+    const DevPot *__restrict__ Pp, double *__restrict__ wq, double *__restrict__ vsync,
+    double *__restrict__ E0, int32_t *__restrict__ fb, int64_t N, double dt, int mode,
+    int32_t step, double tol) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *bufB = bufA + TILE * LDSR;
+    __shared__ float feat[TILE][4];
+    __shared__ float out[TILE][4];
+    const DevPot &P = *Pp;
+    const int t = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * TILE;
+    const int64_t i = row0 + t;
+    const bool own = t < TILE && i < N;
+    double x = nm.pad_r, y = 0, z = 0, hx = 0, hy = 0, hz = 0, r = 1;
+    int32_t fallback = -1;
+    if (own) {
+        x = wq[i];
+        y = wq[N + i];
+        z = wq[2 * N + i];
+        hx = wq[3 * N + i];
+        hy = wq[4 * N + i];
+        hz = wq[5 * N + i];
+        fallback = fb[i];
+        if (mode == 1) {
+            x = x + hx * dt;
+            y = y + hy * dt;
+            z = z + hz * dt;
+        }
+    }
+    if (t < TILE) {
+        float f[4];
+        features(nm, x, y, z, f, r);
+        feat[t][0] = f[0];
+        feat[t][1] = f[1];
+        feat[t][2] = f[2];
+        feat[t][3] = f[3];
+    }
+    __syncthreads();
+    // the tile runs the network unless every orbit in it has fallen back
+    const int any_net = __syncthreads_or(own && fallback < 0);
+    if (any_net) tile_forward<false>(theta, wsw, feat, bufA, bufB, out, row0, N, nullptr, nullptr);
+    if (!own) return;
+    double gx, gy, gz, phi;
+    const double rs = r + nm.r_s;
+    const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
+    if (fallback < 0) {
+        phi = phiH * (1.0 + (double)out[t][0]);
+        gx = gH * (x / r + (double)out[t][1]);
+        gy = gH * (y / r + (double)out[t][2]);
+        gz = gH * (z / r + (double)out[t][3]);
+    }
+    const double vx0 = (mode == 0) ? hx : 0.0, vy0 = (mode == 0) ? hy : 0.0,
+                 vz0 = (mode == 0) ? hz : 0.0;
+    bool switch_now = false;
+    if (fallback < 0) {
+        // energy monitor on the synchronised velocity this force gives
+        const double vx = (mode == 0) ? vx0 : hx - gx * dt / 2.0;
+        const double vy = (mode == 0) ? vy0 : hy - gy * dt / 2.0;
+        const double vz = (mode == 0) ? vz0 : hz - gz * dt / 2.0;
+        const double E = phi + 0.5 * (vx * vx + vy * vy + vz * vz);
+        if (mode == 0) {
+            E0[i] = E;
+        } else if (fabs(E - E0[i]) > tol * fabs(E0[i])) {
+            switch_now = true;
+            fallback = step;
+            fb[i] = step;
+        }
+    }
+    if (fallback >= 0) {
+        double ph;
+        gx = gy = gz = 0.0;
+        const DevComp &c = P.c[0];
+        scf_eval(c, x, y, z, true, ph, gx, gy, gz);
+        (void)switch_now;
+    }
+    if (mode == 0) {
+        wq[3 * N + i] = hx - gx * dt / 2.0;
+        wq[4 * N + i] = hy - gy * dt / 2.0;
+        wq[5 * N + i] = hz - gz * dt / 2.0;
+    } else {
+        wq[i] = x;
+        wq[N + i] = y;
+        wq[2 * N + i] = z;
+        vsync[i] = hx - gx * dt / 2.0;
+        vsync[N + i] = hy - gy * dt / 2.0;
+        vsync[2 * N + i] = hz - gz * dt / 2.0;
+        wq[3 * N + i] = hx - gx * dt;
+        wq[4 * N + i] = hy - gy * dt;
+        wq[5 * N + i] = hz - gz * dt;
+    }
+}
+
+int check_params(const float *theta, const uint16_t *wbf) {
+    return (theta && wbf) ? GAMD_OK : GAMD_ERR_INVALID;
+}
+
+// the tile kernels use 132 KB of dynamic LDS (two padded 128 x 256 BF16
+// activation buffers); raise their limit once per process
+int ensure_lds() {
+    static bool done = false;
+    if (done) return GAMD_OK;
+    const void *ks[3] = {(const void *)k_mlp_forward, (const void *)k_mlp_forward_train,
+                         (const void *)k_surrogate_step};
+    for (const void *k : ks)
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES) !=
+            hipSuccess)
+            return GAMD_ERR_HIP;
+    done = true;
+    return GAMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t gamd_mlp_param_count(void) { return NPARAM; }
+int64_t gamd_mlp_bf16_count(void) { return NBF; }
+
+int64_t gamd_mlp_workspace_bytes(int64_t M) {
+    // Zs, Hs: 2 x 4 x M x 256 BF16; Y, dY: 2 x M x 4 FP32; dZ ping-pong 2 x M x 256 BF16;
+    // loss partials 1024 FP32
+    return 2 * 4 * M * H * 2 + 2 * M * 4 * 4 + 2 * M * H * 2 + 1024 * 4 + 256;
+}
+
+int gamd_mlp_sync_bf16(const float *theta, uint16_t *wbf, void *stream) {
+    if (check_params(theta, wbf)) return GAMD_ERR_INVALID;
+    hipLaunchKernelGGL(k_sync_bf16, dim3(3 * H * H / 256), dim3(256), 0, as_stream(stream), theta,
+                       reinterpret_cast<__bf16 *>(wbf));
+    return launch_status();
+}
+
+int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
+                     const double *q, size_t M, double *phi, double *grad, void *stream) {
+    if (check_params(theta, wbf) || !norm || (M && (!q || !phi || !grad))) return GAMD_ERR_INVALID;
+    if (M == 0) return GAMD_OK;
+    if (ensure_lds()) return GAMD_ERR_HIP;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((M + TILE - 1) / TILE)), dim3(256), LDS_BYTES,
+                       as_stream(stream), theta, reinterpret_cast<const __bf16 *>(wbf), *norm, q,
+                       (int64_t)M, phi, grad);
+    return launch_status();
+}
+
+int gamd_mlp_make_targets(const gamd_mlp_norm *norm, const double *q, const double *phi,
+                          const double *grad, size_t M, float *feat, float *target, void *stream) {
+    if (!norm || (M && (!q || !phi || !grad || !feat || !target))) return GAMD_ERR_INVALID;
+    if (M == 0) return GAMD_OK;
+    hipLaunchKernelGGL(k_make_targets, dim3(grid_for(M, 256)), dim3(256), 0, as_stream(stream),
+                       *norm, q, phi, grad, (int64_t)M, feat, target);
+    return launch_status();
+}
+
+int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *feat,
+                       const float *target, size_t M, float *grad, float *loss, void *workspace,
+                       size_t workspace_bytes, void *stream) {
+    if (check_params(theta, wbf) || !feat || !target || !grad || !workspace) return GAMD_ERR_INVALID;
+    if (M == 0 || M % 64) return GAMD_ERR_INVALID;
+    if ((int64_t)workspace_bytes < gamd_mlp_workspace_bytes((int64_t)M)) return GAMD_ERR_INVALID;
+    hipStream_t s = as_stream(stream);
+    const int64_t Mi = (int64_t)M;
+    unsigned char *ws = static_cast<unsigned char *>(workspace);
+    __bf16 *Zs = reinterpret_cast<__bf16 *>(ws);
+    __bf16 *Hs = Zs + 4 * Mi * H;
+    float *Y = reinterpret_cast<float *>(Hs + 4 * Mi * H);
+    float *dY = Y + Mi * 4;
+    __bf16 *dZa = reinterpret_cast<__bf16 *>(dY + Mi * 4);
+    __bf16 *dZb = dZa + Mi * H;
+    float *partial = reinterpret_cast<float *>(dZb + Mi * H);
+    const __bf16 *wsw = reinterpret_cast<const __bf16 *>(wbf);
+    const __bf16 *wrow = wsw + 3 * H * H;
+    if (ensure_lds()) return GAMD_ERR_HIP;
+    GAMD_HIP_CHECK(hipMemsetAsync(grad, 0, NPARAM * sizeof(float), s));
+    hipLaunchKernelGGL(k_mlp_forward_train, dim3((unsigned)((Mi + TILE - 1) / TILE)), dim3(256),
+                       LDS_BYTES, s, theta, wsw, feat, Mi, Zs, Hs, Y);
+    const unsigned lblocks = 512;
+    hipLaunchKernelGGL(k_loss, dim3(lblocks), dim3(256), 0, s, Y, target, Mi, dY, partial);
+    // output layer
+    hipLaunchKernelGGL(k_outer_reduce, dim3(H, 16), dim3(256), 0, s, Hs + 3 * Mi * H, H, dY, 4, Mi,
+                       grad + OFF_W5);
+    hipLaunchKernelGGL(k_outer_reduce, dim3(1, 16), dim3(256), 0, s, (const __bf16 *)nullptr, 1, dY,
+                       4, Mi, grad + OFF_B5);
+    hipLaunchKernelGGL(k_out_backward_dz, dim3(grid_for(Mi * H, 256)), dim3(256), 0, s, theta, dY,
+                       Zs + 3 * Mi * H, Mi, dZa);
+    // hidden layers 4, 3, 2: dW_l = H_{l-1}^T dZ_l, db_l, dZ_{l-1} = (dZ_l W_l^T) silu'(Z_{l-1})
+    __bf16 *dZ = dZa, *dZn = dZb;
+    const int offW[3] = {OFF_W2, OFF_W3, OFF_W4}, offB[3] = {OFF_B2, OFF_B3, OFF_B4};
+    for (int l = 2; l >= 0; l--) {  // l indexes W_{l+2}
+        const __bf16 *Hin = Hs + (size_t)l * Mi * H;   // H_{l+1}
+        const unsigned ksplit = (Mi % (32 * 16) == 0) ? 16 : 1;  // grad is zeroed above
+        hipLaunchKernelGGL(k_gemm<0>, dim3(H / 64, H / 64, ksplit), dim3(256), 0, s, H, H, (int)Mi, Hin,
+                           (int64_t)1, (int64_t)H, dZ, (int64_t)H, (int64_t)1, grad + offW[l],
+                           (__bf16 *)nullptr, (int64_t)H, (const __bf16 *)nullptr);
+        hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, dZ, Mi, grad + offB[l]);
+        hipLaunchKernelGGL(k_gemm<1>, dim3((unsigned)(Mi / 64), H / 64), dim3(256), 0, s, (int)Mi, H,
+                           H, dZ, (int64_t)H, (int64_t)1, wrow + (size_t)l * H * H, (int64_t)1,
+                           (int64_t)H, (float *)nullptr, dZn, (int64_t)H, Zs + (size_t)l * Mi * H);
+        __bf16 *tmp = dZ;
+        dZ = dZn;
+        dZn = tmp;
+    }
+    // first layer
+    hipLaunchKernelGGL(k_w1_grad, dim3(64), dim3(256), 0, s, feat, dZ, Mi, grad + OFF_W1);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, dZ, Mi, grad + OFF_B1);
+    if (loss) {
+        // sum of the per-block partials into loss[0]
+        GAMD_HIP_CHECK(hipMemsetAsync(loss, 0, sizeof(float), s));
+        hipLaunchKernelGGL(k_outer_reduce, dim3(1, 1), dim3(256), 0, s, (const __bf16 *)nullptr, 1,
+                           partial, 1, (int64_t)lblocks, loss);
+    }
+    return launch_status();
+}
+
+int gamd_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t step, float lr,
+                  float beta1, float beta2, float eps, uint16_t *wbf, void *stream) {
+    if (!theta || !grad || !m || !v || !wbf || step < 1) return GAMD_ERR_INVALID;
+    const float c1 = 1.0f / (1.0f - powf(beta1, (float)step));
+    const float c2 = 1.0f / (1.0f - powf(beta2, (float)step));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(NPARAM, 256)), dim3(256), 0, as_stream(stream), theta,
+                       grad, m, v, (int64_t)NPARAM, lr, beta1, beta2, eps, c1, c2);
+    int rc = launch_status();
+    if (rc) return rc;
+    return gamd_mlp_sync_bf16(theta, wbf, stream);
+}
+
+int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
+                                 const gamd_potential *scf, double *w, double *vsync, double *E0,
+                                 int32_t *fallback_step, size_t N, double dt, int mode,
+                                 int32_t step, double tol, void *stream) {
+    if (check_params(theta, wbf) || !norm || !scf || !w || !vsync || !E0 || !fallback_step)
+        return GAMD_ERR_INVALID;
+    if (scf->dev.n != 1 || scf->dev.c[0].type != GAMD_SCF || scf->dev.c[0].shift_rotate)
+        return GAMD_ERR_INVALID;
+    if (mode != 0 && mode != 1) return GAMD_ERR_INVALID;
+    if (N == 0) return GAMD_OK;
+    if (ensure_lds()) return GAMD_ERR_HIP;
+    hipLaunchKernelGGL(k_surrogate_step, dim3((unsigned)((N + TILE - 1) / TILE)), dim3(256),
+                       LDS_BYTES, as_stream(stream), theta, reinterpret_cast<const __bf16 *>(wbf),
+                       *norm, scf->ddev, w, vsync, E0, fallback_step, (int64_t)N, dt, mode, step,
+                       tol);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -201,6 +201,9 @@ int gamd_potential_create(const gamd_component *components, int ncomponents,
         pot->dev.c[1].type == GAMD_HERNQUIST && pot->dev.c[2].type == GAMD_MIYAMOTONAGAI &&
         pot->dev.c[3].type == GAMD_NFW)
         pot->dev.layout = LAYOUT_MW;
+    if (ncomponents == 1 && pot->dev.all_unshifted && pot->dev.c[0].type == GAMD_SCF &&
+        pot->dev.c[0].nmax == 10 && pot->dev.c[0].lmax == 6)
+        pot->dev.layout = LAYOUT_SCF106;
     // Kernels read the table through a restrict-qualified pointer: with a
     // wave-uniform component index these become scalar loads, where a by-value
     // kernel argument indexed by a runtime component number would be copied to

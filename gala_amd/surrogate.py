@@ -1,0 +1,211 @@
+"""Learned SCF-force surrogate (no gala counterpart): a 4 -> 256 x 4 (SiLU)
+-> 4 MLP trained on forces of the exact SCF expansion, on BF16 matrix cores
+with FP32 accumulation and FP32 master weights, used in a leapfrog whose
+per-orbit energy monitor falls back to the exact SCF force.
+
+Everything runs in the HIP kernels of libgala_amd.so (gamd_mlp_*,
+gamd_surrogate_leapfrog_step); this module owns the device buffers and the
+training loop.
+
+Outputs are in units of the expansion's monopole (the S_000 Hernquist term):
+Phi = Phi_H (1 + o0), grad = g_H (r_hat + o[1:4]), Phi_H = -G M/(r + r_s),
+g_H = G M/(r + r_s)^2, so the network learns a bounded, order-one residual.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _device as dev
+from . import _lib
+
+
+MLPNorm = _lib.MLPNorm
+
+HIDDEN = 256
+PARAM_SHAPES = [("W1", (4, HIDDEN)), ("b1", (HIDDEN,)), ("W2", (HIDDEN, HIDDEN)), ("b2", (HIDDEN,)),
+                ("W3", (HIDDEN, HIDDEN)), ("b3", (HIDDEN,)), ("W4", (HIDDEN, HIDDEN)),
+                ("b4", (HIDDEN,)), ("W5", (HIDDEN, 4)), ("b5", (4,))]
+N_PARAM = sum(int(np.prod(s)) for _, s in PARAM_SHAPES)  # 199,684
+
+
+def init_params(seed=0):
+    """FP32 parameters, flat: W ~ N(0, 1/fan_in) (unit-variance inputs to
+    SiLU), zero biases, a small output layer."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for name, shape in PARAM_SHAPES:
+        if name.startswith("b"):
+            out.append(np.zeros(shape))
+        else:
+            std = 1.0 / math.sqrt(shape[0]) * (0.1 if name == "W5" else 1.0)
+            out.append(rng.normal(0.0, std, shape))
+    return np.concatenate([a.ravel() for a in out]).astype(np.float32)
+
+
+def unpack(theta):
+    """flat parameters -> dict of arrays (W[in][out])."""
+    d, o = {}, 0
+    for name, shape in PARAM_SHAPES:
+        n = int(np.prod(shape))
+        d[name] = np.asarray(theta[o:o + n]).reshape(shape)
+        o += n
+    return d
+
+
+class SCFSurrogate:
+    """The network for one SCF potential (gala_amd.potential.SCFPotential)."""
+
+    def __init__(self, scf_potential, seed=0, L=100.0, rmin=1.0, rmax=200.0):
+        t = dev.require_gpu()
+        lib = _lib.load()
+        self.lib = lib
+        if lib.gamd_mlp_param_count() != N_PARAM:
+            raise RuntimeError("parameter layout mismatch with libgala_amd.so")
+        self.pot = scf_potential
+        self.rmin, self.rmax = rmin, rmax
+        GM = scf_potential.G * scf_potential.parameters["m"]
+        lo, hi = math.log(rmin), math.log(rmax)
+        self.norm = MLPNorm(1.0 / L, 0.5 * (lo + hi), math.sqrt(12.0) / (hi - lo), GM,
+                            scf_potential.parameters["r_s"], 10.0)
+        self.theta = dev.to_dev(init_params(seed), dtype=t.float32)
+        self.wbf = t.empty(lib.gamd_mlp_bf16_count(), dtype=t.bfloat16, device="cuda")
+        self.m = t.zeros_like(self.theta)
+        self.v = t.zeros_like(self.theta)
+        self.grad = t.zeros_like(self.theta)
+        self.step = 0
+        self.history = []
+        self._sync()
+
+    # ------------------------------------------------------------------
+    def _sync(self):
+        _lib.check(self.lib.gamd_mlp_sync_bf16(self.theta.data_ptr(), self.wbf.data_ptr(),
+                                               dev.stream_ptr()), "gamd_mlp_sync_bf16")
+
+    def set_params(self, theta):
+        t = dev.torch()
+        self.theta = dev.to_dev(np.asarray(theta, dtype=np.float32), dtype=t.float32)
+        self._sync()
+
+    def params(self):
+        return dev.to_host(self.theta)
+
+    def sample_points(self, n, seed):
+        """r log-uniform in [rmin, rmax], isotropic directions; (3, n) FP64."""
+        rng = np.random.default_rng(seed)
+        r = np.exp(rng.uniform(math.log(self.rmin), math.log(self.rmax), n))
+        u = rng.normal(size=(3, n))
+        return u / np.linalg.norm(u, axis=0) * r
+
+    def dataset(self, q):
+        """features (M,4) and targets (M,4) on device from the exact FP64 SCF."""
+        t = dev.torch()
+        qd = dev.to_dev(q)
+        M = qd.shape[1]
+        phi = dev.empty((M,))
+        grad = dev.empty((3, M))
+        s = dev.stream_ptr()
+        _lib.check(self.lib.gamd_potential_value(self.pot.c_handle, 0.0,
+                                                               qd.data_ptr(), M, phi.data_ptr(),
+                                                               s), "value")
+        _lib.check(self.lib.gamd_potential_gradient(self.pot.c_handle, 0.0,
+                                                                  qd.data_ptr(), M,
+                                                                  grad.data_ptr(), s), "grad")
+        feat = t.empty((M, 4), dtype=t.float32, device="cuda")
+        target = t.empty((M, 4), dtype=t.float32, device="cuda")
+        _lib.check(self.lib.gamd_mlp_make_targets(C.byref(self.norm), qd.data_ptr(),
+                                                  phi.data_ptr(), grad.data_ptr(), M,
+                                                  feat.data_ptr(), target.data_ptr(), s),
+                   "gamd_mlp_make_targets")
+        return feat, target
+
+    def gradients(self, feat, target):
+        """(loss, grad) of the batch (M a multiple of 64); grad in self.grad."""
+        t = dev.torch()
+        M = feat.shape[0]
+        ws_bytes = int(self.lib.gamd_mlp_workspace_bytes(M))
+        if getattr(self, "_ws", None) is None or self._ws.numel() < ws_bytes:
+            self._ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+            self._loss = t.zeros(1, dtype=t.float32, device="cuda")
+        _lib.check(self.lib.gamd_mlp_gradients(self.theta.data_ptr(), self.wbf.data_ptr(),
+                                               feat.data_ptr(), target.data_ptr(), M,
+                                               self.grad.data_ptr(), self._loss.data_ptr(),
+                                               self._ws.data_ptr(), self._ws.numel(),
+                                               dev.stream_ptr()), "gamd_mlp_gradients")
+        return self._loss, self.grad
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        _lib.check(self.lib.gamd_mlp_adam(self.theta.data_ptr(), self.grad.data_ptr(),
+                                          self.m.data_ptr(), self.v.data_ptr(), self.step, lr,
+                                          beta1, beta2, eps, self.wbf.data_ptr(),
+                                          dev.stream_ptr()), "gamd_mlp_adam")
+
+    def train(self, n_points=16 * 1024 * 1024, batch=32768, epochs=4, lr=2e-3, lr_min=2e-5,
+              seed=1, log=None):
+        """Adam with cosine decay over n_points exact-SCF samples."""
+        t = dev.torch()
+        q = self.sample_points(n_points, seed)
+        feat, target = self.dataset(q)
+        del q
+        n_batches = n_points // batch
+        total = epochs * n_batches
+        gen = t.Generator(device="cuda")
+        gen.manual_seed(seed)
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = t.randperm(n_points, device="cuda", generator=gen)
+            fe, ta = feat[perm].contiguous(), target[perm].contiguous()
+            acc = t.zeros(1, dtype=t.float32, device="cuda")
+            for b in range(n_batches):
+                k = ep * n_batches + b
+                cur = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * k / total))
+                loss, _ = self.gradients(fe[b * batch:(b + 1) * batch],
+                                         ta[b * batch:(b + 1) * batch])
+                acc += loss
+                self.adam(cur)
+            mse = float(acc.item()) / (n_batches * batch * 4)
+            self.history.append(mse)
+            if log:
+                log(f"epoch {ep + 1}/{epochs}: mse {mse:.3e} ({time.perf_counter() - t0:.1f} s)")
+        return self.history
+
+    # ------------------------------------------------------------------
+    def forward(self, q):
+        """Phi (M,) and grad (3, M) from the network at q (3, M)."""
+        qd = dev.to_dev(q)
+        M = qd.shape[1]
+        phi = dev.empty((M,))
+        grad = dev.empty((3, M))
+        _lib.check(self.lib.gamd_mlp_forward(self.theta.data_ptr(), self.wbf.data_ptr(),
+                                             C.byref(self.norm), qd.data_ptr(), M,
+                                             phi.data_ptr(), grad.data_ptr(), dev.stream_ptr()),
+                   "gamd_mlp_forward")
+        return phi, grad
+
+    def leapfrog(self, w0, dt, n_steps, tol=1e-5, progress=None):
+        """Leapfrog of orbits w0 (6, N) with surrogate forces and per-orbit
+        exact-SCF fallback once |dE/E| of the surrogate energy exceeds tol
+        (tol < 0: every orbit on the exact force from the start).
+        Returns dict(w_final (6,N) device, fallback_step (N,) device int32)."""
+        t = dev.torch()
+        w = dev.to_dev(w0).clone()
+        N = w.shape[1]
+        vsync = dev.empty((3, N))
+        E0 = t.zeros(N, dtype=t.float64, device="cuda")
+        fb = t.full((N,), 0 if tol < 0 else -1, dtype=t.int32, device="cuda")
+        s = dev.stream_ptr()
+        fn = self.lib.gamd_surrogate_leapfrog_step
+        args = (self.theta.data_ptr(), self.wbf.data_ptr(), C.byref(self.norm),
+                self.pot.c_handle, w.data_ptr(), vsync.data_ptr(), E0.data_ptr(), fb.data_ptr(),
+                N, float(dt))
+        _lib.check(fn(*args, 0, 0, float(tol), s), "surrogate init")
+        for j in range(1, n_steps + 1):
+            rc = fn(*args, 1, j, float(tol), s)
+            if rc:
+                _lib.check(rc, "surrogate step")
+            if progress and j % 1000 == 0:
+                progress(j)
+        out = t.cat([w[:3], vsync], dim=0) if n_steps > 0 else w
+        return {"w_final": out, "fallback_step": fb}

@@ -144,16 +144,58 @@ int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *pr
                              const double *w0, const double *rel_t, size_t n, double tfinal,
                              double *out, void *stream);
 
-/* ---- SCF surrogate (no reference counterpart) --------------------------- */
-/* A 4 -> 256 x 4 (SiLU) -> 4 MLP standing in for the SCF force; see the
- * gamd_mlp_* entry points.  Weights are BF16 copies of FP32 masters. */
+/* ---- learned SCF-force surrogate --------------------------------------- */
+/* No counterpart in gala: a 4 -> 256 x 4 (SiLU) -> 4 MLP standing in for the
+ * SCF force of scf/src/bfe.cpp:114-189, on BF16 matrix cores with FP32
+ * accumulation.  Parameters: one flat FP32 master buffer theta of
+ * gamd_mlp_param_count() values, layout W1[4][256] b1[256] W2[256][256] b2
+ * W3 b3 W4 b4 W5[256][4] b5[4] (W[in][out]); BF16 copies wbf of
+ * gamd_mlp_bf16_count() values made by gamd_mlp_sync_bf16.
+ * Features of a point q: (x/L, y/L, z/L, (ln r - mu)/sigma); outputs o give
+ * Phi = Phi_H (1 + o0), grad = g_H (r_hat + o[1:4]) with Phi_H = -GM/(r+r_s),
+ * g_H = GM/(r+r_s)^2 (the expansion's monopole). */
 typedef struct {
-    int32_t hidden;          /* 256 */
-    int32_t layers;          /* 4 hidden layers */
-    double in_scale[4];      /* feature scaling: x/L, y/L, z/L, (log r - mu)/sig */
-    double in_shift[4];
-    double out_scale[4];     /* Phi, dPhi/dx, dPhi/dy, dPhi/dz de-normalisation */
-} gamd_mlp_desc;
+    double inv_L;            /* 1 / length scale of the position features */
+    double mu, inv_sigma;    /* ln r feature: (ln r - mu) * inv_sigma */
+    double GM, r_s;          /* monopole scaling of the outputs */
+    double pad_r;            /* any positive radius, used for padding rows */
+} gamd_mlp_norm;
+
+int64_t gamd_mlp_param_count(void);
+int64_t gamd_mlp_bf16_count(void);
+/* bytes of workspace gamd_mlp_gradients needs for a batch of M points */
+int64_t gamd_mlp_workspace_bytes(int64_t M);
+/* BF16 copies (matrix-core fragment order and row-major) of the masters */
+int gamd_mlp_sync_bf16(const float *theta, uint16_t *wbf, void *stream);
+/* inference: q (3,M) FP64 -> phi (M), grad (3,M) FP64 */
+int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
+                     const double *q, size_t M, double *phi, double *grad, void *stream);
+/* training data: exact phi (M) and grad (3,M) at q (3,M) -> features
+ * (M,4) and scaled targets (M,4), FP32 */
+int gamd_mlp_make_targets(const gamd_mlp_norm *norm, const double *q, const double *phi,
+                          const double *grad, size_t M, float *feat, float *target, void *stream);
+/* forward + backward of the mean squared error over a batch (M a multiple
+ * of 64): grad (param_count FP32) overwritten; loss[0] = sum of squared
+ * errors over the batch (nullable) */
+int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *feat,
+                       const float *target, size_t M, float *grad, float *loss, void *workspace,
+                       size_t workspace_bytes, void *stream);
+/* one Adam step on the FP32 masters (bias-corrected, step >= 1), then the
+ * BF16 copies are refreshed */
+int gamd_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t step, float lr,
+                  float beta1, float beta2, float eps, uint16_t *wbf, void *stream);
+/* One leapfrog step of N orbits with surrogate forces (the loop of
+ * leapfrog.pyx:106-116 with the network in place of c_gradient).  w (6,N)
+ * holds positions and half-step velocities between calls; vsync (3,N) the
+ * synchronised velocity after the step; E0 (N) the initial energy (written
+ * by mode 0, the initial half kick); fallback_step (N) is -1 while an orbit
+ * uses the surrogate and the step at which it switched to the exact SCF force
+ * of `scf` (a single unshifted SCF component) when its surrogate energy
+ * Phi + |v|^2/2 drifted by more than tol |E0|. */
+int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
+                                 const gamd_potential *scf, double *w, double *vsync, double *E0,
+                                 int32_t *fallback_step, size_t N, double dt, int mode,
+                                 int32_t step, double tol, void *stream);
 
 /* Library version string (host). */
 const char *gamd_version(void);

@@ -1,0 +1,167 @@
+"""Host-side logic (no GPU): the C ABI library exports what the header
+declares, the gala-shaped API's argument handling, time grids, integrator
+lookup, parameter packing and the seeded workloads."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "gala_amd.h")).read()
+    return sorted(set(re.findall(r"\b(gamd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gala_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libgala_amd.so not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 12
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) <= set(_lib.SIGNATURES), set(syms) - set(_lib.SIGNATURES)
+    assert _lib.load().gamd_version().startswith(b"gala_amd")
+
+
+def test_invalid_arguments_are_rejected_without_gpu_work():
+    """Argument checks happen before any HIP call."""
+    from gala_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libgala_amd.so not built")
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    assert lib.gamd_potential_create(None, 9, ctypes.byref(out)) == -1
+    assert lib.gamd_potential_create(None, -1, ctypes.byref(out)) == -1
+    assert lib.gamd_leapfrog(None, None, 1, 0.0, 1.0, 1, None, 1, None, None, None) == -1
+    assert lib.gamd_dop853(None, None, 1, None, None, None, 1e-8, 1e-10, 0, 0, 0.0, None, 0, None,
+                           None, None, None, None, 0, None) == -1
+    assert lib.gamd_mockstream_leapfrog(None, None, None, 0, 0.0, 1.0, None, None, 0, 0.0, None,
+                                        None) == -1
+    assert lib.gamd_potential_destroy(None) == 0
+
+
+def test_parse_time_specification():
+    from gala_amd.integrate import parse_time_specification as p
+    t = p(dt=1.0, n_steps=10)
+    assert len(t) == 11 and t[0] == 0.0 and t[-1] == 10.0
+    t = p(dt=-1.0, n_steps=4000)
+    assert len(t) == 4001 and t[-1] == -4000.0
+    t = p(dt=0.1, n_steps=5, t1=2.0)
+    np.testing.assert_allclose(t, 2.0 + np.cumsum(np.append([0.0], np.full(5, 0.1))))
+    t = p(dt=0.5, t1=0.0, t2=2.0)
+    np.testing.assert_array_equal(t, [0.0, 0.5, 1.0, 1.5])
+    t = p(dt=-0.5, t1=2.0, t2=0.2)
+    assert t[0] == 2.0 and t[-1] == 0.2
+    t = p(n_steps=5, t1=0.0, t2=1.0)
+    np.testing.assert_array_equal(t, np.linspace(0, 1, 5))
+    arr = np.array([0.0, 0.3, 1.0])
+    np.testing.assert_array_equal(p(t=arr), arr)
+    with pytest.raises(ValueError):
+        p(dt=1.0, t1=5.0, t2=0.0)
+    with pytest.raises(ValueError):
+        p()
+
+
+def test_get_integrator():
+    from gala_amd.integrate import DOPRI853Integrator, LeapfrogIntegrator, get_integrator
+    assert get_integrator("Leapfrog") is LeapfrogIntegrator
+    assert get_integrator("dop853") is DOPRI853Integrator
+    assert get_integrator("DOPRI853") is DOPRI853Integrator
+    assert get_integrator(LeapfrogIntegrator) is LeapfrogIntegrator
+    with pytest.raises(ValueError):
+        get_integrator("euler")
+
+
+def test_parameter_packing():
+    from gala_amd import _lib
+    from gala_amd.potential import (CompositePotential, HernquistPotential,
+                                    MilkyWayLikePotential, MiyamotoNagaiPotential, NFWPotential,
+                                    PlummerPotential, SCFPotential)
+    from gala_amd.units import G
+    h = HernquistPotential(5e9, 1.0)
+    c = h._components()[0]
+    assert c["type"] == _lib.HERNQUIST and c["pars"] == [G, 5e9, 1.0] and c["shift_rotate"] == 0
+    assert HernquistPotential(m=5e9, c=1.0).parameters == h.parameters
+    mn = MiyamotoNagaiPotential(m=6.8e10, a=3.0, b=0.28, origin=[1.0, 0, 0])
+    c = mn._components()[0]
+    assert c["pars"] == [G, 6.8e10, 3.0, 0.28] and c["shift_rotate"] == 1
+    nfw = NFWPotential(m=5.4e11, r_s=15.6)
+    assert nfw._components()[0]["pars"] == [G, 5.4e11, 15.6, 1.0, 1.0, 1.0]
+    with pytest.raises(NotImplementedError):
+        NFWPotential(m=1e12, r_s=20.0, c=0.8)
+    with pytest.raises(TypeError):
+        PlummerPotential(m=1.0)
+    mw = MilkyWayLikePotential()
+    types = [c["type"] for c in mw._components()]
+    assert types == [_lib.HERNQUIST, _lib.HERNQUIST, _lib.MIYAMOTONAGAI, _lib.NFW]
+    comp = CompositePotential()
+    comp["a"] = h
+    comp["b"] = mn
+    assert len(comp._components()) == 2
+    both = h + mn
+    assert isinstance(both, CompositePotential) and len(both._components()) == 2
+    S = np.zeros((3, 2, 2))
+    S[0, 0, 0] = 1.0
+    scf = SCFPotential(m=1e12, r_s=20.0, Snlm=S)
+    c = scf._components()[0]
+    assert c["type"] == _lib.SCF and c["S"].shape == (3, 2, 2) and np.all(c["T"] == 0)
+    arr, keep = _lib.make_components(mw._components())
+    assert arr[3].type == _lib.NFW and arr[3].pars[2] == 15.6
+
+
+def test_workloads_are_seeded_and_shaped():
+    from gala_amd import workloads
+
+    def vc(q):
+        return np.full(q.shape[1], 0.2)
+    w0a, dt, n = workloads.composite_leapfrog(vc, n=64)
+    w0b, _, _ = workloads.composite_leapfrog(vc, n=64)
+    assert w0a.shape == (6, 64) and dt == 1.0 and n == 10000
+    np.testing.assert_array_equal(w0a, w0b)
+    r = np.linalg.norm(w0a[:3], axis=0)
+    assert np.all((r >= 5) & (r <= 30))
+    # velocity is tangential apart from the 10 km/s noise
+    vr = np.sum(w0a[:3] * w0a[3:], axis=0) / r
+    assert np.max(np.abs(vr)) < 5 * 10 * 1.0227e-3
+    w0, T = workloads.composite_dop853(vc, n=1000)
+    assert np.all((T >= 100) & (T <= 10000))
+    r = np.linalg.norm(w0[:3], axis=0)
+    assert np.all((r >= 1) & (r <= 200))
+    S, Tc = workloads.scf_coefficients()
+    assert S.shape == (11, 7, 7) and S[0, 0, 0] == 1.0 and np.all(Tc[:, :, 0] == 0)
+    for l in range(7):
+        for m in range(l + 1, 7):
+            assert np.all(S[:, l, m] == 0) and np.all(Tc[:, l, m] == 0)
+    order = workloads.cost_order(w0, T, lambda q: -0.01 * q)
+    assert sorted(order) == list(range(1000))
+
+
+def test_units():
+    from gala_amd import units
+    assert abs(units.G - 4.498502151469553e-12) < 1e-26
+    assert abs(units.KMS_TO_KPCMYR - 1.0227121650456959e-3) < 1e-15
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+
+
+def test_no_cpu_fallback_in_product():
+    """The product never imports the oracle and has no CPU path."""
+    pkg = os.path.join(ROOT, "gala_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.lower() or f == "__init__.py", f
