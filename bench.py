@@ -139,6 +139,110 @@ def bench_dop853(args):
     return out, (w0, T)
 
 
+def _energy(pot, w):
+    """exact H = Phi + |v|^2/2 of device states w (6, N), on device."""
+    from gala_amd import _device as dev
+    from gala_amd import _lib
+    lib = _lib.load()
+    N = w.shape[1]
+    out = dev.empty((N,))
+    _lib.check(lib.gamd_hamiltonian_energy(pot.c_handle, 0.0, w.contiguous().data_ptr(), N,
+                                           out.data_ptr(), dev.stream_ptr()),
+               "gamd_hamiltonian_energy")
+    return out
+
+
+def _timed(fn, stream):
+    """(result, wall s, HIP-event ms on the launch stream) of one call."""
+    import torch
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    res = fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t0, e0.elapsed_time(e1)
+
+
+def bench_scf(args):
+    """Exact SCF leapfrog (nmax 10, lmax 6, m 1e12, r_s 20) and the learned
+    surrogate on the same 262,144 orbits x 10,000 steps of dt = 0.5 Myr."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import roofline, workloads
+    from gala_amd.integrate import leapfrog
+    from gala_amd.potential import SCFPotential
+    from gala_amd.surrogate import SCFSurrogate
+
+    S, T = workloads.scf_coefficients()
+    pot = SCFPotential(m=1e12, r_s=20.0, Snlm=S, Tnlm=T)
+    n = args.scf_n or 262144
+    w0, dt, n_steps = workloads.scf_leapfrog(pot.circular_velocity, n=n, seed=3)
+    n_steps = args.scf_steps or n_steps
+    w0d = dev.to_dev(w0)
+    stream = torch.cuda.current_stream()
+    E0 = _energy(pot, w0d)
+
+    def rel_drift(w):
+        return dev.to_host(((_energy(pot, w) - E0) / E0).abs())
+
+    leapfrog(pot, w0d, dt, 10, save=False)  # warm-up
+    (_, wf), wall, ev = _timed(lambda: leapfrog(pot, w0d, dt, n_steps, save=False), stream)
+    d_exact = rel_drift(wf)
+    exact = {"workload": "scf_leapfrog_exact", "n_orbits": n, "n_steps": n_steps, "dt": dt,
+             "value": n * n_steps / wall, "unit": "orbit-steps/s", "wall_s": wall,
+             "kernel_ms": ev, "dE_over_E": {"median": float(np.median(d_exact)),
+                                            "p99": float(np.percentile(d_exact, 99))},
+             "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)",
+                          "achieved": n * n_steps * roofline.FLOP_SCF_LEAPFROG_STEP / (ev / 1e3) / 1e12,
+                          "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}}
+    exact["roofline"]["frac"] = exact["roofline"]["achieved"] / roofline.PEAK_FP64_TFLOPS
+    log(f"exact SCF leapfrog: {exact['value']:.3e} orbit-steps/s")
+
+    sur = SCFSurrogate(pot, seed=0)
+    (hist), t_train, _ = _timed(lambda: sur.train(n_points=args.train_points, batch=32768,
+                                                  epochs=args.train_epochs, lr=2e-3,
+                                                  lr_min=2e-5, seed=1, log=log), stream)
+    qs = sur.sample_points(1 << 20, seed=99)
+    _, g_net = sur.forward(qs)
+    g_ex = pot.gradient(qs)
+    e_f = np.linalg.norm(dev.to_host(g_net) - g_ex, axis=0) / np.linalg.norm(g_ex, axis=0)
+    sur.leapfrog(w0d, dt, 2, tol=1.0)  # warm-up
+    res, wall_s, ev_s = _timed(lambda: sur.leapfrog(w0d, dt, n_steps, tol=args.tol), stream)
+    fb = dev.to_host(res["fallback_step"])
+    d_sur = rel_drift(res["w_final"])
+    n_short = max(1, n_steps // 10)
+    res2, wall_p, ev_p = _timed(lambda: sur.leapfrog(w0d, dt, n_short, tol=1e300), stream)
+    d_pure = rel_drift(res2["w_final"])
+    _, wall_e2, _ = _timed(lambda: leapfrog(pot, w0d, dt, n_short, save=False), stream)
+    surrogate = {
+        "workload": "scf_leapfrog_surrogate", "network": "4 -> 4 x 256 SiLU -> 4, BF16 MFMA",
+        "train_points": args.train_points, "train_epochs": args.train_epochs,
+        "train_s": t_train, "train_final_mse": float(hist[-1]),
+        "force_rel_err": {"median": float(np.median(e_f)), "p90": float(np.percentile(e_f, 90)),
+                          "p99": float(np.percentile(e_f, 99)), "points": int(qs.shape[1])},
+        "fallback_tol": args.tol, "value": n * n_steps / wall_s, "unit": "orbit-steps/s",
+        "wall_s": wall_s, "fraction_fallen_back": float(np.mean(fb >= 0)),
+        "dE_over_E": {"median": float(np.median(d_sur)), "p99": float(np.percentile(d_sur, 99))},
+        "surrogate_only": {"n_steps": n_short, "value": n * n_short / wall_p,
+                           "exact_value_same_steps": n * n_short / wall_e2,
+                           "dE_over_E": {"median": float(np.median(d_pure)),
+                                         "p99": float(np.percentile(d_pure, 99))},
+                           "roofline": {"bound": "mfma", "pipe": "bf16 MFMA",
+                                        "achieved": n * n_short * roofline.FLOP_MLP_FORWARD
+                                        / (ev_p / 1e3) / 1e12,
+                                        "peak": roofline.PEAK_BF16_TFLOPS, "unit": "TFLOP/s"}},
+        "speedup_vs_exact": (n * n_steps / wall_s) / exact["value"],
+    }
+    rl = surrogate["surrogate_only"]["roofline"]
+    rl["frac"] = rl["achieved"] / rl["peak"]
+    log(f"surrogate: {surrogate['value']:.3e} orbit-steps/s, fallback "
+        f"{surrogate['fraction_fallen_back']:.3f}, force err median {np.median(e_f):.2e}")
+    return exact, surrogate
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +254,14 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU baseline sample: every k-th orbit")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="skip the exact-SCF and surrogate measurements")
+    ap.add_argument("--scf-n", type=int, default=0, help="override the SCF orbit count")
+    ap.add_argument("--scf-steps", type=int, default=0, help="override the SCF step count")
+    ap.add_argument("--train-points", type=int, default=16 * 1024 * 1024)
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--tol", type=float, default=1e-5,
+                    help="surrogate energy-drift bound for the exact-SCF fallback")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -160,6 +272,8 @@ def main():
     if args.workload != "composite_dop853":
         sys.exit(f"unknown workload {args.workload}")
     out, (w0, T) = bench_dop853(args)
+    if not args.no_scf:
+        out["scf_exact"], out["scf_surrogate"] = bench_scf(args)
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline_dop853(w0, T, args.cpu_stride)

@@ -31,8 +31,12 @@ class Component(C.Structure):
 
 class MLPNorm(C.Structure):
     """gamd_mlp_norm (include/gala_amd.h)."""
-    _fields_ = [("inv_L", C.c_double), ("mu", C.c_double), ("inv_sigma", C.c_double),
-                ("GM", C.c_double), ("r_s", C.c_double), ("pad_r", C.c_double)]
+    _fields_ = [("mu", C.c_double), ("inv_sigma", C.c_double),
+                ("lnr0", C.c_double), ("inv_dlnr", C.c_double), ("pad_r", C.c_double),
+                ("scale", C.c_void_p)]
+
+
+MLP_KNOTS = 32  # GAMD_MLP_KNOTS
 
 
 # every symbol include/gala_amd.h declares, with its signature

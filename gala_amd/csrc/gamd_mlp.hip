@@ -6,13 +6,16 @@
 // potential/scf/src/bfe.cpp:114-189.
 //
 // Network, per point q (FP64 in, FP64 out):
-//   features  f = (x/L, y/L, z/L, (ln r - mu)/sigma)
+//   features  f = (x/r, y/r, z/r, (ln r - mu)/sigma): direction and log
+//             radius, so small radii are resolved as finely as large ones
 //   hidden    h1 = silu(f W1 + b1), h_l = silu(h_{l-1} W_l + b_l), l = 2..4
 //   output    o = h4 W5 + b5   (4 values)
-//   physics   Phi  = Phi_H(r) (1 + o0),         Phi_H = -G M / (r + r_s)
-//             grad = g_H(r) (r_hat + (o1, o2, o3)), g_H = G M / (r + r_s)^2
-// so the network learns the departure from the expansion's monopole (the
-// S_000 Hernquist term) in units of the local monopole scale.
+//   physics   Phi = s_phi(r) o0,  grad = s_grad(r) (o1, o2, o3)
+// with s_phi, s_grad radial scale profiles (piecewise linear in ln r), so
+// every output is of order one at every radius: with these expansions the
+// force varies by orders of magnitude over 1-200 kpc and departs from the
+// monopole (S_000) force by factors of several, so neither raw nor
+// monopole-relative targets train well.
 //
 // Matrix-core mapping (gfx950, v_mfma_f32_16x16x32_bf16; operand layout
 // A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8);
@@ -64,10 +67,26 @@ __device__ __forceinline__ float frombf(__bf16 b) { return (float)b; }
 __device__ __forceinline__ void features(const gamd_mlp_norm &nm, double x, double y, double z,
                                          float *f, double &r) {
     r = sqrt(x * x + y * y + z * z);
-    f[0] = (float)(x * nm.inv_L);
-    f[1] = (float)(y * nm.inv_L);
-    f[2] = (float)(z * nm.inv_L);
+    const double ir = 1.0 / r;
+    f[0] = (float)(x * ir);
+    f[1] = (float)(y * ir);
+    f[2] = (float)(z * ir);
     f[3] = (float)((log(r) - nm.mu) * nm.inv_sigma);
+}
+
+// the output scales at radius r: linear interpolation in ln r between the
+// profile knots, constant beyond the end knots (numpy.interp's convention)
+__device__ __forceinline__ void out_scales(const gamd_mlp_norm &nm, double r, double &sphi,
+                                           double &sgrad) {
+    double u = (log(r) - nm.lnr0) * nm.inv_dlnr;
+    if (!(u > 0.0)) u = 0.0;  // also maps NaN (r = 0 with a zero knot spacing) to the first knot
+    if (u > (double)(GAMD_MLP_KNOTS - 1)) u = (double)(GAMD_MLP_KNOTS - 1);
+    int k = (int)u;
+    k = k > GAMD_MLP_KNOTS - 2 ? GAMD_MLP_KNOTS - 2 : k;
+    const double f = u - (double)k;
+    const double *sp = nm.scale, *sg = nm.scale + GAMD_MLP_KNOTS;
+    sphi = sp[k] + f * (sp[k + 1] - sp[k]);
+    sgrad = sg[k] + f * (sg[k + 1] - sg[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,12 +230,12 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ t
     if (t < TILE) {
         const int64_t i = row0 + t;
         if (i < M) {
-            const double rs = r + nm.r_s;
-            const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
-            phi[i] = phiH * (1.0 + (double)out[t][0]);
-            grad[i] = gH * (x / r + (double)out[t][1]);
-            grad[M + i] = gH * (y / r + (double)out[t][2]);
-            grad[2 * M + i] = gH * (z / r + (double)out[t][3]);
+            double sphi, sgrad;
+            out_scales(nm, r, sphi, sgrad);
+            phi[i] = sphi * (double)out[t][0];
+            grad[i] = sgrad * (double)out[t][1];
+            grad[M + i] = sgrad * (double)out[t][2];
+            grad[2 * M + i] = sgrad * (double)out[t][3];
         }
     }
 }
@@ -441,12 +460,12 @@ __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
     features(nm, x, y, z, f, r);
 #pragma unroll
     for (int c = 0; c < 4; c++) feat[i * 4 + c] = f[c];
-    const double rs = r + nm.r_s;
-    const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
-    target[i * 4 + 0] = (float)(phi[i] / phiH - 1.0);
-    target[i * 4 + 1] = (float)(grad[i] / gH - x / r);
-    target[i * 4 + 2] = (float)(grad[M + i] / gH - y / r);
-    target[i * 4 + 3] = (float)(grad[2 * M + i] / gH - z / r);
+    double sphi, sgrad;
+    out_scales(nm, r, sphi, sgrad);
+    target[i * 4 + 0] = (float)(phi[i] / sphi);
+    target[i * 4 + 1] = (float)(grad[i] / sgrad);
+    target[i * 4 + 2] = (float)(grad[M + i] / sgrad);
+    target[i * 4 + 3] = (float)(grad[2 * M + i] / sgrad);
 }
 
 // ---------------------------------------------------------------------------
@@ -503,13 +522,13 @@ __global__ __launch_bounds__(256) void k_surrogate_step(
     if (any_net) tile_forward<false>(theta, wsw, feat, bufA, bufB, out, row0, N, nullptr, nullptr);
     if (!own) return;
     double gx, gy, gz, phi;
-    const double rs = r + nm.r_s;
-    const double phiH = -nm.GM / rs, gH = nm.GM / (rs * rs);
     if (fallback < 0) {
-        phi = phiH * (1.0 + (double)out[t][0]);
-        gx = gH * (x / r + (double)out[t][1]);
-        gy = gH * (y / r + (double)out[t][2]);
-        gz = gH * (z / r + (double)out[t][3]);
+        double sphi, sgrad;
+        out_scales(nm, r, sphi, sgrad);
+        phi = sphi * (double)out[t][0];
+        gx = sgrad * (double)out[t][1];
+        gy = sgrad * (double)out[t][2];
+        gz = sgrad * (double)out[t][3];
     }
     const double vx0 = (mode == 0) ? hx : 0.0, vy0 = (mode == 0) ? hy : 0.0,
                  vz0 = (mode == 0) ? hz : 0.0;
@@ -550,6 +569,14 @@ __global__ __launch_bounds__(256) void k_surrogate_step(
         wq[4 * N + i] = hy - gy * dt;
         wq[5 * N + i] = hz - gz * dt;
     }
+}
+
+int check_norm(const gamd_mlp_norm *nm) {
+    if (!nm || !nm->scale) return GAMD_ERR_INVALID;
+    if (!(nm->inv_dlnr > 0.0) || !isfinite(nm->inv_dlnr) || !isfinite(nm->lnr0) ||
+        !(nm->pad_r > 0.0))
+        return GAMD_ERR_INVALID;
+    return GAMD_OK;
 }
 
 int check_params(const float *theta, const uint16_t *wbf) {
@@ -593,7 +620,8 @@ int gamd_mlp_sync_bf16(const float *theta, uint16_t *wbf, void *stream) {
 
 int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
                      const double *q, size_t M, double *phi, double *grad, void *stream) {
-    if (check_params(theta, wbf) || !norm || (M && (!q || !phi || !grad))) return GAMD_ERR_INVALID;
+    if (check_params(theta, wbf) || check_norm(norm) || (M && (!q || !phi || !grad)))
+        return GAMD_ERR_INVALID;
     if (M == 0) return GAMD_OK;
     if (ensure_lds()) return GAMD_ERR_HIP;
     hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((M + TILE - 1) / TILE)), dim3(256), LDS_BYTES,
@@ -604,7 +632,8 @@ int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_nor
 
 int gamd_mlp_make_targets(const gamd_mlp_norm *norm, const double *q, const double *phi,
                           const double *grad, size_t M, float *feat, float *target, void *stream) {
-    if (!norm || (M && (!q || !phi || !grad || !feat || !target))) return GAMD_ERR_INVALID;
+    if (check_norm(norm) || (M && (!q || !phi || !grad || !feat || !target)))
+        return GAMD_ERR_INVALID;
     if (M == 0) return GAMD_OK;
     hipLaunchKernelGGL(k_make_targets, dim3(grid_for(M, 256)), dim3(256), 0, as_stream(stream),
                        *norm, q, phi, grad, (int64_t)M, feat, target);
@@ -687,7 +716,8 @@ int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const 
                                  const gamd_potential *scf, double *w, double *vsync, double *E0,
                                  int32_t *fallback_step, size_t N, double dt, int mode,
                                  int32_t step, double tol, void *stream) {
-    if (check_params(theta, wbf) || !norm || !scf || !w || !vsync || !E0 || !fallback_step)
+    if (check_params(theta, wbf) || check_norm(norm) || !scf || !w || !vsync || !E0 ||
+        !fallback_step)
         return GAMD_ERR_INVALID;
     if (scf->dev.n != 1 || scf->dev.c[0].type != GAMD_SCF || scf->dev.c[0].shift_rotate)
         return GAMD_ERR_INVALID;

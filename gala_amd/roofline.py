@@ -41,3 +41,26 @@ PEAK_HBM_GBS = 8000.0
 
 def dop853_flops(naccpt, nrejct):
     return naccpt * FLOP_DOP853_ACCEPTED + nrejct * FLOP_DOP853_REJECTED
+
+
+def copy_calibration(n_doubles=1 << 25, reps=5):
+    """Run a plain elementwise kernel of known traffic (torch.neg on
+    n_doubles FP64 values: 8 n bytes read, 8 n bytes written per launch) so
+    that rocprofv3's FETCH_SIZE / WRITE_SIZE counters can be scaled to
+    bytes: factor = known bytes / counter bytes of this kernel."""
+    import torch
+    src = torch.ones(n_doubles, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {reps} launches of torch.neg over {n_doubles} doubles "
+          f"({8 * n_doubles} bytes read, {8 * n_doubles} bytes written each)", flush=True)
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "calibrate":
+        copy_calibration()
+    else:
+        sys.exit("usage: python -m gala_amd.roofline calibrate")

@@ -7,9 +7,11 @@ Everything runs in the HIP kernels of libgala_amd.so (gamd_mlp_*,
 gamd_surrogate_leapfrog_step); this module owns the device buffers and the
 training loop.
 
-Outputs are in units of the expansion's monopole (the S_000 Hernquist term):
-Phi = Phi_H (1 + o0), grad = g_H (r_hat + o[1:4]), Phi_H = -G M/(r + r_s),
-g_H = G M/(r + r_s)^2, so the network learns a bounded, order-one residual.
+Outputs are scaled by radial profiles of the exact potential: Phi =
+s_phi(r) o0, grad = s_grad(r) o[1:4], with s_phi and s_grad the RMS of the
+exact Phi and of one exact gradient component over shells in ln r (32 knots
+between rmin and rmax, hat-function weights), so every target is of order
+one at every radius.
 """
 import ctypes as C
 import math
@@ -57,7 +59,7 @@ def unpack(theta):
 class SCFSurrogate:
     """The network for one SCF potential (gala_amd.potential.SCFPotential)."""
 
-    def __init__(self, scf_potential, seed=0, L=100.0, rmin=1.0, rmax=200.0):
+    def __init__(self, scf_potential, seed=0, rmin=1.0, rmax=200.0):
         t = dev.require_gpu()
         lib = _lib.load()
         self.lib = lib
@@ -65,10 +67,10 @@ class SCFSurrogate:
             raise RuntimeError("parameter layout mismatch with libgala_amd.so")
         self.pot = scf_potential
         self.rmin, self.rmax = rmin, rmax
-        GM = scf_potential.G * scf_potential.parameters["m"]
         lo, hi = math.log(rmin), math.log(rmax)
-        self.norm = MLPNorm(1.0 / L, 0.5 * (lo + hi), math.sqrt(12.0) / (hi - lo), GM,
-                            scf_potential.parameters["r_s"], 10.0)
+        self.scale = self._scale_profile(lo, hi)
+        self.norm = MLPNorm(0.5 * (lo + hi), math.sqrt(12.0) / (hi - lo), lo,
+                            (_lib.MLP_KNOTS - 1) / (hi - lo), 10.0, self.scale.data_ptr())
         self.theta = dev.to_dev(init_params(seed), dtype=t.float32)
         self.wbf = t.empty(lib.gamd_mlp_bf16_count(), dtype=t.bfloat16, device="cuda")
         self.m = t.zeros_like(self.theta)
@@ -79,6 +81,38 @@ class SCFSurrogate:
         self._sync()
 
     # ------------------------------------------------------------------
+    def _exact(self, q):
+        """exact Phi (M,) and grad (3, M) on device at q (3, M) (device or host)."""
+        qd = dev.to_dev(q)
+        M = qd.shape[1]
+        phi = dev.empty((M,))
+        grad = dev.empty((3, M))
+        s = dev.stream_ptr()
+        _lib.check(self.lib.gamd_potential_value(self.pot.c_handle, 0.0, qd.data_ptr(), M,
+                                                 phi.data_ptr(), s), "gamd_potential_value")
+        _lib.check(self.lib.gamd_potential_gradient(self.pot.c_handle, 0.0, qd.data_ptr(), M,
+                                                    grad.data_ptr(), s),
+                   "gamd_potential_gradient")
+        return qd, phi, grad
+
+    def _scale_profile(self, lo, hi, n=1 << 16, seed=20240):
+        """(2, K) device table: RMS of the exact Phi and of one exact gradient
+        component around each of the K knots in ln r (hat-function weights)."""
+        K = _lib.MLP_KNOTS
+        q = self.sample_points(n, seed)
+        _, phi, grad = self._exact(q)
+        phi, grad = dev.to_host(phi), dev.to_host(grad)
+        u = (np.log(np.linalg.norm(q, axis=0)) - lo) * ((K - 1) / (hi - lo))
+        wts = np.maximum(0.0, 1.0 - np.abs(u[:, None] - np.arange(K)[None, :]))
+        den = np.maximum(wts.sum(axis=0), 1e-300)
+        sp = np.sqrt((wts * (phi ** 2)[:, None]).sum(axis=0) / den)
+        sg = np.sqrt((wts * (np.sum(grad ** 2, axis=0) / 3.0)[:, None]).sum(axis=0) / den)
+        table = np.stack([sp, sg])
+        if not np.all(np.isfinite(table)) or np.any(table <= 0.0):
+            raise RuntimeError("degenerate output scale profile (zero potential or force "
+                               "over a shell)")
+        return dev.to_dev(table)
+
     def _sync(self):
         _lib.check(self.lib.gamd_mlp_sync_bf16(self.theta.data_ptr(), self.wbf.data_ptr(),
                                                dev.stream_ptr()), "gamd_mlp_sync_bf16")
@@ -101,17 +135,9 @@ class SCFSurrogate:
     def dataset(self, q):
         """features (M,4) and targets (M,4) on device from the exact FP64 SCF."""
         t = dev.torch()
-        qd = dev.to_dev(q)
+        qd, phi, grad = self._exact(q)
         M = qd.shape[1]
-        phi = dev.empty((M,))
-        grad = dev.empty((3, M))
         s = dev.stream_ptr()
-        _lib.check(self.lib.gamd_potential_value(self.pot.c_handle, 0.0,
-                                                               qd.data_ptr(), M, phi.data_ptr(),
-                                                               s), "value")
-        _lib.check(self.lib.gamd_potential_gradient(self.pot.c_handle, 0.0,
-                                                                  qd.data_ptr(), M,
-                                                                  grad.data_ptr(), s), "grad")
         feat = t.empty((M, 4), dtype=t.float32, device="cuda")
         target = t.empty((M, 4), dtype=t.float32, device="cuda")
         _lib.check(self.lib.gamd_mlp_make_targets(C.byref(self.norm), qd.data_ptr(),

@@ -151,14 +151,18 @@ int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *pr
  * gamd_mlp_param_count() values, layout W1[4][256] b1[256] W2[256][256] b2
  * W3 b3 W4 b4 W5[256][4] b5[4] (W[in][out]); BF16 copies wbf of
  * gamd_mlp_bf16_count() values made by gamd_mlp_sync_bf16.
- * Features of a point q: (x/L, y/L, z/L, (ln r - mu)/sigma); outputs o give
- * Phi = Phi_H (1 + o0), grad = g_H (r_hat + o[1:4]) with Phi_H = -GM/(r+r_s),
- * g_H = GM/(r+r_s)^2 (the expansion's monopole). */
+ * Features of a point q: (x/r, y/r, z/r, (ln r - mu)/sigma); outputs o give
+ * Phi = s_phi(r) o0 and grad = s_grad(r) o[1:4], where s_phi and s_grad are
+ * radial scale profiles (typically the RMS of the exact Phi and of one
+ * gradient component over shells), piecewise linear in ln r between
+ * GAMD_MLP_KNOTS knots at ln r = lnr0 + k / inv_dlnr and constant beyond the
+ * end knots. */
+#define GAMD_MLP_KNOTS 32
 typedef struct {
-    double inv_L;            /* 1 / length scale of the position features */
     double mu, inv_sigma;    /* ln r feature: (ln r - mu) * inv_sigma */
-    double GM, r_s;          /* monopole scaling of the outputs */
+    double lnr0, inv_dlnr;   /* knot positions of the output scale profiles */
     double pad_r;            /* any positive radius, used for padding rows */
+    const double *scale;     /* DEVICE [2][GAMD_MLP_KNOTS]: s_phi knots, then s_grad knots */
 } gamd_mlp_norm;
 
 int64_t gamd_mlp_param_count(void);
@@ -191,7 +195,8 @@ int gamd_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t s
  * by mode 0, the initial half kick); fallback_step (N) is -1 while an orbit
  * uses the surrogate and the step at which it switched to the exact SCF force
  * of `scf` (a single unshifted SCF component) when its surrogate energy
- * Phi + |v|^2/2 drifted by more than tol |E0|. */
+ * Phi + |v|^2/2 drifted by more than tol |E0|.  mode 0 is the initial half
+ * kick (and sets E0), mode 1 one drift + kick. */
 int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
                                  const gamd_potential *scf, double *w, double *vsync, double *E0,
                                  int32_t *fallback_step, size_t N, double dt, int mode,

@@ -9,6 +9,8 @@ b4 W5[256][4] b5[4] with W[in][out].
 Forward:  h1 = silu(f W1 + b1); h_l = silu(h_{l-1} W_l + b_l), l = 2..4;
           o = h4 W5 + b5.
 Loss:     L = sum_m sum_o (o_mo - t_mo)^2 / M.
+Physics:  Phi = s_phi(r) o0, grad = s_grad(r) o[1:4] with the radial scale
+          profiles interpolated linearly in ln r (out_scales).
 """
 import numpy as np
 
@@ -66,23 +68,26 @@ def loss_and_grad(theta, feat, target):
     return L, pack(g)
 
 
-def features_and_targets(q, phi, grad, L, mu, inv_sigma, GM, r_s):
-    """The scaled training pairs: q (3,M), exact phi (M,), grad (3,M)."""
+def out_scales(r, lnr0, inv_dlnr, scale):
+    """radial output scales: scale (2, K) knots at ln r = lnr0 + k / inv_dlnr,
+    linear in ln r between knots, constant beyond the ends."""
+    knots = lnr0 + np.arange(scale.shape[1]) / inv_dlnr
+    lr = np.log(r)
+    return np.interp(lr, knots, scale[0]), np.interp(lr, knots, scale[1])
+
+
+def features_and_targets(q, phi, grad, mu, inv_sigma, lnr0, inv_dlnr, scale):
+    """The scaled training pairs: q (3,M), exact phi (M,), grad (3,M).
+    Features (x/r, y/r, z/r, (ln r - mu) inv_sigma)."""
     r = np.sqrt(np.sum(q * q, axis=0))
-    feat = np.stack([q[0] / L, q[1] / L, q[2] / L, (np.log(r) - mu) * inv_sigma], axis=1)
-    rs = r + r_s
-    phiH = -GM / rs
-    gH = GM / rs ** 2
-    target = np.stack([phi / phiH - 1.0, grad[0] / gH - q[0] / r, grad[1] / gH - q[1] / r,
-                       grad[2] / gH - q[2] / r], axis=1)
+    feat = np.stack([q[0] / r, q[1] / r, q[2] / r, (np.log(r) - mu) * inv_sigma], axis=1)
+    sp, sg = out_scales(r, lnr0, inv_dlnr, scale)
+    target = np.stack([phi / sp, grad[0] / sg, grad[1] / sg, grad[2] / sg], axis=1)
     return feat, target
 
 
-def physical(q, y, GM, r_s):
+def physical(q, y, lnr0, inv_dlnr, scale):
     """network outputs -> (Phi, grad) at q."""
     r = np.sqrt(np.sum(q * q, axis=0))
-    rs = r + r_s
-    phi = -GM / rs * (1.0 + y[:, 0])
-    gH = GM / rs ** 2
-    grad = gH * (q / r + y[:, 1:4].T)
-    return phi, grad
+    sp, sg = out_scales(r, lnr0, inv_dlnr, scale)
+    return sp * y[:, 0], sg * y[:, 1:4].T

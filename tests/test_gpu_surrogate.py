@@ -31,16 +31,15 @@ def test_forward_matches_oracle():
     q = sur.sample_points(1000, seed=5)
     phi, grad = sur.forward(q)
     phi, grad = dev.to_host(phi), dev.to_host(grad)
-    nm = sur.norm
-    feat, _ = OM.features_and_targets(q, np.ones(1000), np.ones((3, 1000)), 1 / nm.inv_L, nm.mu,
-                                      nm.inv_sigma, nm.GM, nm.r_s)
+    nm, sc = sur.norm, dev.to_host(sur.scale)
+    feat, _ = OM.features_and_targets(q, np.ones(1000), np.ones((3, 1000)), nm.mu,
+                                      nm.inv_sigma, nm.lnr0, nm.inv_dlnr, sc)
     y, _ = OM.forward(sur.params(), feat)
-    phi_o, grad_o = OM.physical(q, y, nm.GM, nm.r_s)
-    r = np.linalg.norm(q, axis=0)
-    gH = nm.GM / (r + nm.r_s) ** 2
+    phi_o, grad_o = OM.physical(q, y, nm.lnr0, nm.inv_dlnr, sc)
+    sp, sg = OM.out_scales(np.linalg.norm(q, axis=0), nm.lnr0, nm.inv_dlnr, sc)
     # compare in network-output units
-    assert np.max(np.abs(phi / phi_o - 1.0)) < 2e-2 * (1 + np.max(np.abs(y[:, 0])))
-    assert np.max(np.abs(grad - grad_o) / gH) < 2e-2 * (1 + np.max(np.abs(y[:, 1:])))
+    assert np.max(np.abs(phi - phi_o) / sp) < 2e-2 * (1 + np.max(np.abs(y[:, 0])))
+    assert np.max(np.abs(grad - grad_o) / sg) < 2e-2 * (1 + np.max(np.abs(y[:, 1:])))
 
 
 def test_make_targets_matches_oracle():
@@ -49,9 +48,9 @@ def test_make_targets_matches_oracle():
     q = sur.sample_points(2000, seed=6)
     feat, target = sur.dataset(q)
     orc = [O.scf(1e12, 20.0, S, T)]
-    nm = sur.norm
-    fo, to = OM.features_and_targets(q, O.value(orc, q), O.gradient(orc, q), 1 / nm.inv_L, nm.mu,
-                                     nm.inv_sigma, nm.GM, nm.r_s)
+    nm, sc = sur.norm, dev.to_host(sur.scale)
+    fo, to = OM.features_and_targets(q, O.value(orc, q), O.gradient(orc, q), nm.mu,
+                                     nm.inv_sigma, nm.lnr0, nm.inv_dlnr, sc)
     np.testing.assert_allclose(dev.to_host(feat), fo, rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(dev.to_host(target), to, rtol=1e-5, atol=1e-6)
 
@@ -91,8 +90,8 @@ def test_training_beats_monopole_and_falls_back():
     g_net = dev.to_host(g_net)
     g_ex = pot.gradient(q)
     r = np.linalg.norm(q, axis=0)
-    nm = sur.norm
-    g_mono = nm.GM / (r + nm.r_s) ** 2 * q / r
+    GM, r_s = pot.G * pot.parameters["m"], pot.parameters["r_s"]
+    g_mono = GM / (r + r_s) ** 2 * q / r  # the expansion's S_000 (Hernquist) term alone
     e_net = np.linalg.norm(g_net - g_ex, axis=0) / np.linalg.norm(g_ex, axis=0)
     e_mono = np.linalg.norm(g_mono - g_ex, axis=0) / np.linalg.norm(g_ex, axis=0)
     print(f"median force error: network {np.median(e_net):.2e}, monopole {np.median(e_mono):.2e}")

@@ -40,9 +40,27 @@ def test_physical_scaling_inverts_targets():
     q = rng.uniform(-50, 50, (3, 100))
     phi = -rng.uniform(0.1, 1.0, 100)
     grad = rng.normal(size=(3, 100)) * 0.01
-    GM, rs = 4.5, 20.0
-    feat, target = OM.features_and_targets(q, phi, grad, 100.0, 2.65, 0.65, GM, rs)
-    p2, g2 = OM.physical(q, target, GM, rs)
+    scale = rng.uniform(0.5, 2.0, (2, 32))
+    lnr0, inv_dlnr = 0.0, 31.0 / np.log(200.0)
+    feat, target = OM.features_and_targets(q, phi, grad, 2.65, 0.65, lnr0, inv_dlnr, scale)
+    p2, g2 = OM.physical(q, target, lnr0, inv_dlnr, scale)
     np.testing.assert_allclose(p2, phi, rtol=1e-12)
     np.testing.assert_allclose(g2, grad, rtol=1e-10, atol=1e-14)
     assert feat.shape == (100, 4)
+
+
+def test_out_scales_interpolate_and_clamp():
+    """knot values are reproduced at the knots, midpoints average their
+    neighbours, and radii outside the knot span take the end values."""
+    scale = np.stack([np.arange(1.0, 33.0), np.arange(1.0, 33.0) ** 2])
+    lnr0, inv_dlnr = np.log(2.0), 2.0
+    knots_r = np.exp(lnr0 + np.arange(32) / inv_dlnr)
+    sp, sg = OM.out_scales(knots_r, lnr0, inv_dlnr, scale)
+    np.testing.assert_allclose(sp, scale[0], rtol=1e-12)
+    np.testing.assert_allclose(sg, scale[1], rtol=1e-12)
+    mid = np.exp(lnr0 + (np.arange(31) + 0.5) / inv_dlnr)
+    sp, _ = OM.out_scales(mid, lnr0, inv_dlnr, scale)
+    np.testing.assert_allclose(sp, 0.5 * (scale[0][:-1] + scale[0][1:]), rtol=1e-12)
+    sp, sg = OM.out_scales(np.array([1e-3, 1e9]), lnr0, inv_dlnr, scale)
+    assert sp[0] == scale[0][0] and sp[1] == scale[0][-1]
+    assert sg[0] == scale[1][0] and sg[1] == scale[1][-1]
