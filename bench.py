@@ -166,6 +166,70 @@ def _timed(fn, stream):
     return res, time.perf_counter() - t0, e0.elapsed_time(e1)
 
 
+def bench_leapfrog(args):
+    """Plain FP64 leapfrog in the Milky-Way-like composite: the 1024-orbit,
+    10,000-step, dt = 1 Myr case with every step saved (gala's default
+    save_all), and 1,048,576 orbits x 1,000 steps keeping the final state
+    only and saving every 16th step; the oracle's OpenMP leapfrog on the
+    host cores beside each."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import roofline, workloads
+    from gala_amd.integrate import leapfrog
+    from gala_amd.potential import MilkyWayLikePotential
+    from oracle import oracle as O
+
+    pot = MilkyWayLikePotential()
+    orc = O.milky_way_like()
+    stream = torch.cuda.current_stream()
+    out = {}
+    w0, dt, n_steps = workloads.composite_leapfrog(pot.circular_velocity, n=1024, seed=0)
+    w0d = dev.to_dev(w0)
+    leapfrog(pot, w0d, dt, 10)
+    _, wall, ev = _timed(lambda: leapfrog(pot, w0d, dt, n_steps, save_stride=1), stream)
+    cpu = None
+    if not args.no_cpu:
+        c0 = time.perf_counter()
+        O.leapfrog(orc, w0, np.arange(n_steps + 1) * dt, save_all=True, fast=True)
+        cpu = 1024 * n_steps / (time.perf_counter() - c0)
+    out["n1024_save_all"] = {"n_orbits": 1024, "n_steps": n_steps, "dt": dt, "save_stride": 1,
+                             "value": 1024 * n_steps / wall, "unit": "orbit-steps/s",
+                             "kernel_ms": ev, "cpu_value": cpu,
+                             "cpu_cores": int(O.num_threads(fast=True))}
+    n = args.lf_n or 1048576
+    steps = 1000
+    w0, _, _ = workloads.composite_leapfrog(pot.circular_velocity, n=n, seed=0)
+    w0d = dev.to_dev(w0)
+    leapfrog(pot, w0d, dt, 10, save=False)
+    _, wall_f, ev_f = _timed(lambda: leapfrog(pot, w0d, dt, steps, save=False), stream)
+    _, wall_s, ev_s = _timed(lambda: leapfrog(pot, w0d, dt, steps, save_stride=16), stream)
+    ns = min(n, 65536)
+    cpu = None
+    if not args.no_cpu:
+        c0 = time.perf_counter()
+        O.leapfrog(orc, np.ascontiguousarray(w0[:, :ns]), np.arange(steps + 1) * dt,
+                   save_all=False, fast=True)
+        cpu = ns * steps / (time.perf_counter() - c0)
+    flops = n * steps * roofline.FLOP_LEAPFROG_STEP
+    saved = 6 * (steps // 16 + 1) * n * 8
+    out["n1M_final"] = {
+        "n_orbits": n, "n_steps": steps, "dt": dt, "value": n * steps / wall_f,
+        "unit": "orbit-steps/s", "kernel_ms": ev_f,
+        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": flops / (ev_f / 1e3) / 1e12,
+                     "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s"},
+        "cpu_value": cpu, "cpu_cores": int(O.num_threads(fast=True)),
+        "cpu_sample": f"first {ns} orbits, {steps} steps, final state only"}
+    out["n1M_final"]["roofline"]["frac"] = out["n1M_final"]["roofline"]["achieved"] / \
+        roofline.PEAK_FP64_TFLOPS
+    out["n1M_final"]["gpu_over_cpu"] = out["n1M_final"]["value"] / cpu if cpu else None
+    out["n1M_stride16"] = {"n_orbits": n, "n_steps": steps, "save_stride": 16,
+                           "value": n * steps / wall_s, "unit": "orbit-steps/s", "kernel_ms": ev_s,
+                           "hbm_write_GBps": saved / (ev_s / 1e3) / 1e9}
+    log(f"leapfrog composite: 1024x{n_steps} save_all {out['n1024_save_all']['value']:.3e}/s, "
+        f"{n}x{steps} {out['n1M_final']['value']:.3e}/s (CPU {cpu}/s)")
+    return out
+
+
 def bench_scf(args):
     """Exact SCF leapfrog (nmax 10, lmax 6, m 1e12, r_s 20) and the learned
     surrogate on the same 262,144 orbits x 10,000 steps of dt = 0.5 Myr."""
@@ -256,6 +320,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the exact-SCF and surrogate measurements")
+    ap.add_argument("--no-leapfrog", action="store_true",
+                    help="skip the composite leapfrog measurements")
+    ap.add_argument("--lf-n", type=int, default=0, help="override the large leapfrog orbit count")
     ap.add_argument("--scf-n", type=int, default=0, help="override the SCF orbit count")
     ap.add_argument("--scf-steps", type=int, default=0, help="override the SCF step count")
     ap.add_argument("--train-points", type=int, default=16 * 1024 * 1024)
@@ -272,6 +339,8 @@ def main():
     if args.workload != "composite_dop853":
         sys.exit(f"unknown workload {args.workload}")
     out, (w0, T) = bench_dop853(args)
+    if not args.no_leapfrog:
+        out["leapfrog_composite"] = bench_leapfrog(args)
     if not args.no_scf:
         out["scf_exact"], out["scf_surrogate"] = bench_scf(args)
     if not args.no_cpu:

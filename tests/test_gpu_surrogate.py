@@ -83,8 +83,8 @@ def test_training_beats_monopole_and_falls_back():
     from gala_amd import _device as dev
     from gala_amd.integrate import leapfrog
     sur, pot, S, T = _surrogate(0)
-    hist = sur.train(n_points=1 << 20, batch=8192, epochs=3, lr=2e-3, lr_min=1e-4, seed=1)
-    assert hist[-1] < 0.2 * hist[0]
+    hist = sur.train(n_points=1 << 21, batch=8192, epochs=3, lr=2e-3, lr_min=1e-4, seed=1)
+    assert hist[-1] < 0.1 * hist[0]
     q = sur.sample_points(20000, seed=99)
     _, g_net = sur.forward(q)
     g_net = dev.to_host(g_net)
@@ -102,8 +102,13 @@ def test_training_beats_monopole_and_falls_back():
     _, wf = leapfrog(pot, w0, dt, 200, save=False)
     assert np.all(dev.to_host(res["fallback_step"]) == 0)
     assert np.max(state_rel_err(dev.to_host(res["w_final"]), dev.to_host(wf))) < 1e-10
-    # the monitor: with a loose bound nothing falls back, with a tight one some do
-    loose = sur.leapfrog(w0, dt, 200, tol=1.0)
-    assert np.all(dev.to_host(loose["fallback_step"]) < 0)
-    tight = sur.leapfrog(w0, dt, 200, tol=1e-9)
-    assert np.mean(dev.to_host(tight["fallback_step"]) > 0) > 0.5
+    # the monitor: a loose bound lets most orbits stay on the network (those
+    # that leave the trained radii fall back), a tight one sends most to the
+    # exact force; until its first fallback an orbit evolves identically in
+    # both runs, so the tight run never switches later than the loose one
+    fb_loose = dev.to_host(sur.leapfrog(w0, dt, 200, tol=1.0)["fallback_step"])
+    fb_tight = dev.to_host(sur.leapfrog(w0, dt, 200, tol=1e-9)["fallback_step"])
+    assert np.mean(fb_loose >= 0) < 0.1
+    assert np.mean(fb_tight > 0) > 0.5
+    both = fb_loose >= 0
+    assert np.all((fb_tight[both] >= 0) & (fb_tight[both] <= fb_loose[both]))
