@@ -53,6 +53,7 @@ SIGNATURES = {
     "gamd_potential_d2_dr2": (_int, [_vp, _d, _vp, _sz, _vp, _vp]),
     "gamd_hamiltonian_energy": (_int, [_vp, _d, _vp, _sz, _vp, _vp]),
     "gamd_leapfrog": (_int, [_vp, _vp, _sz, _d, _d, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "gamd_ruth4": (_int, [_vp, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp, _vp]),
     "gamd_dop853": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _d, _d, _i64, _i64, _d, _vp, _sz,
                            _vp, _vp, _vp, _vp, _vp, _int, _vp]),
     "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _d, _vp,

@@ -19,6 +19,7 @@
 //     keeps the tail short); the loop ends when the queue is exhausted for
 //     every lane of the wave, which every wave reaches.
 #include <math.h>
+#include <stdlib.h>
 
 #include "dop853_tableau.h"
 #include "gamd_host.hpp"
@@ -45,6 +46,7 @@ struct DopArgs {
     int64_t *stats;
     const int32_t *order;
     unsigned long long *queue;
+    int wide;              // finish the last orbits of a wave in 16-lane rows
 };
 
 #define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
@@ -100,19 +102,365 @@ __device__ double hinit(const DevPot &P, double x, const double *y, double posne
     return sign_d(h, posneg);
 }
 
+// ---------------------------------------------------------------------------
+// Two lane layouts share one step-attempt body, templated on NC, the number
+// of state components a lane holds:
+//   NC = 6: one lane owns one orbit (the bulk of the run);
+//   NC = 1: the "wide tail": once the queue is drained and a wave holds at
+//           most four unfinished orbits, each orbit is spread over one DPP
+//           row of 16 lanes, lane c of the row holding component c % 6 (lanes
+//           12..15 duplicate components 3..5).  The stage combinations, error
+//           terms and dense updates then cost one component's arithmetic
+//           instead of six; every lane of the row evaluates the force at the
+//           row's broadcast position.  Sums over components are formed from
+//           broadcast terms in the one-lane order, so both layouts produce
+//           bit-identical results.
+// The tail matters because the uneven workload's longest orbits need ~5x the
+// average number of steps and end up running alone on their SIMDs.
+
+constexpr int WIDE_ROW = 16;   // lanes per orbit in the wide tail (one DPP row)
+constexpr int WIDE_ORBITS = 4; // orbits per wave in the wide tail
+
+template <int SRC>
+__device__ __forceinline__ double row_bcast(double v) {
+    // DPP row_newbcast:SRC: every lane of a 16-lane row reads lane SRC of the row
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + SRC, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + SRC, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// DPP row_shl:3: lane i of a row reads lane i + 3 (lanes 13..15 keep their own)
+__device__ __forceinline__ double row_shl3(double v) {
+    const int lo0 = __double2loint(v), hi0 = __double2hiint(v);
+    const int lo = __builtin_amdgcn_update_dpp(lo0, lo0, 0x103, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(hi0, hi0, 0x103, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// f = (p, -grad Phi(q)) for the lane's components
+template <bool SCF, int LAY>
+__device__ __forceinline__ void rhs_n(const DevPot &P, int, const double (&y)[6], double (&f)[6]) {
+    rhs<SCF, LAY>(P, y, f);
+}
+template <bool SCF, int LAY>
+__device__ __forceinline__ void rhs_n(const DevPot &P, int comp, const double (&y)[1],
+                                      double (&f)[1]) {
+    const double qx = row_bcast<0>(y[0]), qy = row_bcast<1>(y[0]), qz = row_bcast<2>(y[0]);
+    const double vp = row_shl3(y[0]);  // the velocity partner of components 0..2
+    double gx, gy, gz;
+    pot_grad_l<SCF, LAY>(P, qx, qy, qz, gx, gy, gz);
+    const double g = comp == 3 ? gx : (comp == 4 ? gy : gz);
+    f[0] = comp < 3 ? vp : -g;
+}
+
+// the six per-component values of an orbit, in component order
+__device__ __forceinline__ void all6(const double (&v)[6], double (&o)[6]) { FOR6 o[c] = v[c]; }
+__device__ __forceinline__ void all6(const double (&v)[1], double (&o)[6]) {
+    o[0] = row_bcast<0>(v[0]);
+    o[1] = row_bcast<1>(v[0]);
+    o[2] = row_bcast<2>(v[0]);
+    o[3] = row_bcast<3>(v[0]);
+    o[4] = row_bcast<4>(v[0]);
+    o[5] = row_bcast<5>(v[0]);
+}
+
+// per-orbit scalars of the integration (wave-uniform within a wide row)
+struct DopState {
+    int64_t orb;
+    double x, xend, h, hmax, posneg, facold, hlamb;
+    int64_t nstep, naccpt, nrejct, nfcn, oidx;
+    int reject, iasti, nonsti;
+};
+
+#define FORN _Pragma("unroll") for (int c = 0; c < NC; c++)
+
+// One step attempt (dopcor basic step, dop853.cpp:339-648) for the lane's NC
+// components.  Returns 0 to go on, else the orbit's final idid.
+template <bool SCF, bool DENSE, int LAY, int NC>
+__device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, int comp, DopState &s,
+                                           double (&y)[NC], double (&k1)[NC]) {
+    static_assert(NC == 6 || !DENSE, "dense output runs with one lane per orbit");
+    const double uround = 2.220446049250313e-16;  // np.finfo(float).eps (dop853.pyx:165)
+    const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
+    if (s.nstep > a.nmax) return -2;
+    if (0.1 * fabs(s.h) <= fabs(s.x) * uround) return -3;
+    const bool last = (s.x + 1.01 * s.h - s.xend) * s.posneg > 0.0;
+    const double h = last ? s.xend - s.x : s.h;
+    s.h = h;
+    s.nstep++;
+    double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], k7[NC], k8[NC], k9[NC], k10[NC], yy[NC];
+    FORN yy[c] = y[c] + h * T::A[1][0] * k1[c];
+    rhs_n<SCF, LAY>(P, comp, yy, k2);
+    FORN yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k3);
+    FORN yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k4);
+    FORN yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k5);
+    FORN yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k6);
+    FORN yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
+                             T::A[6][5] * k6[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k7);
+    FORN yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
+                             T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k8);
+    FORN yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
+                             T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k9);
+    FORN yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
+                             T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
+                             T::A[9][8] * k9[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k10);
+    FORN yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
+                             T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
+                             T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
+    const double xph = s.x + h;
+    FORN yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
+                             T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
+                             T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
+    rhs_n<SCF, LAY>(P, comp, yy, k3);  // stage 12 reuses k3 (:408)
+    s.nfcn += 11;
+    FORN {
+        k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
+                T::B[8] * k9[c] + T::B[9] * k10[c] + T::B[10] * k2[c] + T::B[11] * k3[c];
+        k5[c] = y[c] + h * k4[c];
+    }
+    // error estimation (:416-444), n = 6; one divide serves both estimates
+    double e2[NC], e1[NC];
+    FORN {
+        const double sk = a.atol + a.rtol * max_d(fabs(y[c]), fabs(k5[c]));
+        const double isk = 1.0 / sk;
+        double erri = k4[c] - T::BHH[0] * k1[c] - T::BHH[8] * k9[c] - T::BHH[11] * k3[c];
+        e2[c] = erri * isk;
+        erri = T::E5[0] * k1[c] + T::E5[5] * k6[c] + T::E5[6] * k7[c] + T::E5[7] * k8[c] +
+               T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
+        e1[c] = erri * isk;
+    }
+    double q2[6], q1[6];
+    all6(e2, q2);
+    all6(e1, q1);
+    double err = 0.0, err2 = 0.0;
+    FOR6 {
+        err2 += q2[c] * q2[c];
+        err += q1[c] * q1[c];
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * 6.0));
+    // step size factor (:447-452), beta = 0: err^(1/8) as three square
+    // roots (~330 cycles of latency where pow's log/exp pair is ~1000)
+    const double fac11 = sqrt(sqrt(sqrt(err)));
+    const double fac = max_d(facc2, min_d(facc1, fac11 / safe));
+    double hnew = h / fac;
+    int code = 0;
+    if (err <= 1.0) {
+        // accepted (:454-637)
+        s.facold = max_d(err, 1.0e-4);
+        s.naccpt++;
+        rhs_n<SCF, LAY>(P, comp, k5, k4);  // FSAL
+        s.nfcn++;
+        // stiffness test (:463-486); returns -4 on the first detection
+        if (a.nstiff > 0 && (!(s.naccpt % a.nstiff) || s.iasti > 0)) {
+            double dn[NC], dd[NC];
+            FORN {
+                dn[c] = k4[c] - k3[c];
+                dd[c] = k5[c] - yy[c];
+            }
+            double qn[6], qd[6];
+            all6(dn, qn);
+            all6(dd, qd);
+            double stnum = 0.0, stden = 0.0;
+            FOR6 {
+                stnum += qn[c] * qn[c];
+                stden += qd[c] * qd[c];
+            }
+            if (stden > 0.0) s.hlamb = h * sqrt(stnum / stden);
+            if (s.hlamb > 6.1) {
+                s.nonsti = 0;
+                s.iasti++;
+                code = -4;
+            } else {
+                s.nonsti++;
+                if (s.nonsti == 6) s.iasti = 0;
+            }
+        }
+        if (code == 0) {
+            if constexpr (DENSE) {
+                // dense output coefficients (:492-582) and sampling (:584-612)
+                double r1[6], r2[6], r3[6], r4[6], r5[6], r6[6], r7[6], r8[6];
+                FOR6 {
+                    r1[c] = y[c];
+                    const double ydiff = k5[c] - y[c];
+                    r2[c] = ydiff;
+                    const double bspl = h * k1[c] - ydiff;
+                    r3[c] = bspl;
+                    r4[c] = ydiff - h * k4[c] - bspl;
+                    r5[c] = T::D[0][0] * k1[c] + T::D[0][5] * k6[c] + T::D[0][6] * k7[c] +
+                            T::D[0][7] * k8[c] + T::D[0][8] * k9[c] + T::D[0][9] * k10[c] +
+                            T::D[0][10] * k2[c] + T::D[0][11] * k3[c];
+                    r6[c] = T::D[1][0] * k1[c] + T::D[1][5] * k6[c] + T::D[1][6] * k7[c] +
+                            T::D[1][7] * k8[c] + T::D[1][8] * k9[c] + T::D[1][9] * k10[c] +
+                            T::D[1][10] * k2[c] + T::D[1][11] * k3[c];
+                    r7[c] = T::D[2][0] * k1[c] + T::D[2][5] * k6[c] + T::D[2][6] * k7[c] +
+                            T::D[2][7] * k8[c] + T::D[2][8] * k9[c] + T::D[2][9] * k10[c] +
+                            T::D[2][10] * k2[c] + T::D[2][11] * k3[c];
+                    r8[c] = T::D[3][0] * k1[c] + T::D[3][5] * k6[c] + T::D[3][6] * k7[c] +
+                            T::D[3][7] * k8[c] + T::D[3][8] * k9[c] + T::D[3][9] * k10[c] +
+                            T::D[3][10] * k2[c] + T::D[3][11] * k3[c];
+                }
+                FOR6 yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
+                                         T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
+                                         T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
+                                         T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
+                rhs<SCF, LAY>(P, yy, k10);
+                FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
+                                         T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
+                                         T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
+                                         T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
+                rhs<SCF, LAY>(P, yy, k2);
+                FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
+                                         T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
+                                         T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
+                                         T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
+                rhs<SCF, LAY>(P, yy, k3);
+                s.nfcn += 3;
+                FOR6 {
+                    r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
+                                 T::D[0][14] * k2[c] + T::D[0][15] * k3[c]);
+                    r6[c] = h * (r6[c] + T::D[1][12] * k4[c] + T::D[1][13] * k10[c] +
+                                 T::D[1][14] * k2[c] + T::D[1][15] * k3[c]);
+                    r7[c] = h * (r7[c] + T::D[2][12] * k4[c] + T::D[2][13] * k10[c] +
+                                 T::D[2][14] * k2[c] + T::D[2][15] * k3[c]);
+                    r8[c] = h * (r8[c] + T::D[3][12] * k4[c] + T::D[3][13] * k10[c] +
+                                 T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
+                }
+                const double x0 = s.x, x1 = s.x + h;
+                const int64_t so = a.n_out * a.N;
+                while (s.oidx < a.n_out) {
+                    const double to = a.t_out[s.oidx];
+                    if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
+                        const double sv = (to - s.x) / h, s1 = 1.0 - sv;
+                        FOR6 a.w_out[c * so + s.oidx * a.N + s.orb] =
+                            r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
+                            sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
+                        s.oidx++;
+                    } else {
+                        break;
+                    }
+                }
+            }
+            FORN {
+                k1[c] = k4[c];
+                y[c] = k5[c];
+            }
+            s.x = xph;
+            if (last) {
+                code = 1;
+            } else {
+                if (fabs(hnew) > s.hmax) hnew = s.posneg * s.hmax;
+                if (s.reject) hnew = s.posneg * min_d(fabs(hnew), fabs(h));
+                s.reject = 0;
+            }
+        }
+    } else {
+        // rejected (:638-645)
+        hnew = h / min_d(facc1, fac11 / safe);
+        s.reject = 1;
+        if (s.naccpt >= 1) s.nrejct++;
+    }
+    if (code == 0) s.h = hnew;
+    return code;
+}
+
+__device__ __forceinline__ void finish_stats(const DopArgs &a, const DopState &s, int code) {
+    if (a.status) a.status[s.orb] = code;
+    if (a.stats) {
+        a.stats[4 * s.orb + 0] = s.nstep;
+        a.stats[4 * s.orb + 1] = s.naccpt;
+        a.stats[4 * s.orb + 2] = s.nrejct;
+        a.stats[4 * s.orb + 3] = s.nfcn;
+    }
+}
+
+template <typename V>
+__device__ __forceinline__ V from_lane(V v, int src) {
+    return __shfl(v, src);
+}
+
+// The wide tail of one wave: the (at most WIDE_ORBITS) orbits still active
+// move to one 16-lane row each and are integrated to the end.  Called with
+// every lane of the wave converged; the queue is drained.
+template <bool SCF, int LAY>
+__device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, bool active,
+                                       const DopState &s6, const double (&y6)[6],
+                                       const double (&k16)[6]) {
+    const int lane = threadIdx.x & 63;
+    const int row = lane / WIDE_ROW, c = lane % WIDE_ROW;
+    const int comp = c < 12 ? c % 6 : (c - 9 < 5 ? c - 9 : 5);
+    unsigned long long m = __ballot(active);
+    for (int i = 0; i < row; i++) m &= m - 1ull;  // the row-th active lane
+    bool ract = m != 0ull;
+    const int src = ract ? __ffsll((long long)m) - 1 : lane;
+    double y[1] = {0.0}, k1[1] = {0.0};
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double vy = from_lane(y6[j], src), vk = from_lane(k16[j], src);
+        if (comp == j) {
+            y[0] = vy;
+            k1[0] = vk;
+        }
+    }
+    DopState s;
+    s.orb = from_lane(s6.orb, src);
+    s.x = from_lane(s6.x, src);
+    s.xend = from_lane(s6.xend, src);
+    s.h = from_lane(s6.h, src);
+    s.hmax = from_lane(s6.hmax, src);
+    s.posneg = from_lane(s6.posneg, src);
+    s.facold = from_lane(s6.facold, src);
+    s.hlamb = from_lane(s6.hlamb, src);
+    s.nstep = from_lane(s6.nstep, src);
+    s.naccpt = from_lane(s6.naccpt, src);
+    s.nrejct = from_lane(s6.nrejct, src);
+    s.nfcn = from_lane(s6.nfcn, src);
+    s.oidx = from_lane(s6.oidx, src);
+    s.reject = from_lane(s6.reject, src);
+    s.iasti = from_lane(s6.iasti, src);
+    s.nonsti = from_lane(s6.nonsti, src);
+    // every row ends: each attempt advances nstep, and nmax bounds it
+    while (__any(ract)) {
+        if (ract) {
+            const int code = dop_attempt<SCF, false, LAY, 1>(P, a, comp, s, y, k1);
+            if (code != 0) {
+                if (c < 6) a.w_final[c * a.N + s.orb] = y[0];
+                if (c == 0) finish_stats(a, s, code);
+                ract = false;
+            }
+        }
+    }
+}
+
 template <bool SCF, bool DENSE, int LAY>
 __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a) {
     const DevPot &P = *Pp;
     const int lane = threadIdx.x & 63;
-    const double uround = 2.220446049250313e-16;  // np.finfo(float).eps (dop853.pyx:165)
-    const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
 
     bool active = false, exhausted = false;
-    int64_t orb = 0;
-    double x = 0, xend = 0, h = 0, hmax = 0, posneg = 1, facold = 1e-4, hlamb = 0;
-    int64_t nstep = 0, naccpt = 0, nrejct = 0, nfcn = 0, oidx = 0;
-    int reject = 0, iasti = 0, nonsti = 0;
+    DopState s;
+    s.orb = 0;
+    s.x = s.xend = s.h = s.hmax = 0.0;
+    s.posneg = 1.0;
+    s.facold = 1e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.nrejct = s.nfcn = s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = 0;
     double y[6], k1[6];
+    FOR6 {
+        y[c] = 0.0;
+        k1[c] = 0.0;
+    }
 
     while (true) {
         // ---- refill idle lanes from the queue (one atomic per wave) ----
@@ -127,22 +475,24 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                 const unsigned long long below = want & ((1ull << lane) - 1ull);
                 const unsigned long long slot = base + (unsigned long long)__popcll(below);
                 if (slot < (unsigned long long)a.N) {
-                    orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
+                    int64_t orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
                     if (orb < 0 || orb >= a.N) orb = (int64_t)slot;  // malformed order: fall back
+                    s.orb = orb;
                     FOR6 y[c] = a.w0[c * a.N + orb];
-                    x = a.t0[orb];
-                    xend = a.t1[orb];
-                    h = a.h0 ? a.h0[orb] : 0.0;
-                    hmax = a.hmax > 0.0 ? a.hmax : fabs(xend - x);  // dop853.cpp:787-788, 312
-                    posneg = sign_d(1.0, xend - x);
-                    facold = 1.0e-4;
-                    hlamb = 0.0;
-                    nstep = naccpt = nrejct = 0;
-                    nfcn = 2;
-                    oidx = 0;
-                    reject = iasti = nonsti = 0;
+                    s.x = a.t0[orb];
+                    s.xend = a.t1[orb];
+                    s.h = a.h0 ? a.h0[orb] : 0.0;
+                    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(s.xend - s.x);  // dop853.cpp:787-788, 312
+                    s.posneg = sign_d(1.0, s.xend - s.x);
+                    s.facold = 1.0e-4;
+                    s.hlamb = 0.0;
+                    s.nstep = s.naccpt = s.nrejct = 0;
+                    s.nfcn = 2;
+                    s.oidx = 0;
+                    s.reject = s.iasti = s.nonsti = 0;
                     rhs<SCF, LAY>(P, y, k1);
-                    if (h == 0.0) h = hinit<SCF, LAY>(P, x, y, posneg, k1, hmax, a.rtol, a.atol);
+                    if (s.h == 0.0)
+                        s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
                     active = true;
                 } else {
                     exhausted = true;
@@ -150,203 +500,20 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
             }
         }
         if (!__any(active)) break;
+        if (!DENSE && a.wide) {
+            // every idle lane has found the queue empty; with few orbits
+            // left, spread them over rows and finish them there
+            if (__popcll(__ballot(active)) <= WIDE_ORBITS) {
+                wide_tail<SCF, LAY>(P, a, active, s, y, k1);
+                break;
+            }
+        }
         if (!active) continue;
 
-        // ---- one step attempt (dopcor basic step, dop853.cpp:339-648) ----
-        int code = 0;  // nonzero: the orbit ends with this idid
-        bool last = false;
-        if (nstep > a.nmax) {
-            code = -2;
-        } else if (0.1 * fabs(h) <= fabs(x) * uround) {
-            code = -3;
-        }
-        if (code == 0) {
-            if ((x + 1.01 * h - xend) * posneg > 0.0) {
-                h = xend - x;
-                last = true;
-            }
-            nstep++;
-            double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], yy[6];
-            FOR6 yy[c] = y[c] + h * T::A[1][0] * k1[c];
-            rhs<SCF, LAY>(P, yy, k2);
-            FOR6 yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
-            rhs<SCF, LAY>(P, yy, k3);
-            FOR6 yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
-            rhs<SCF, LAY>(P, yy, k4);
-            FOR6 yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
-            rhs<SCF, LAY>(P, yy, k5);
-            FOR6 yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
-            rhs<SCF, LAY>(P, yy, k6);
-            FOR6 yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
-                                     T::A[6][5] * k6[c]);
-            rhs<SCF, LAY>(P, yy, k7);
-            FOR6 yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
-                                     T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
-            rhs<SCF, LAY>(P, yy, k8);
-            FOR6 yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
-                                     T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
-            rhs<SCF, LAY>(P, yy, k9);
-            FOR6 yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
-                                     T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
-                                     T::A[9][8] * k9[c]);
-            rhs<SCF, LAY>(P, yy, k10);
-            FOR6 yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
-                                     T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
-                                     T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
-            rhs<SCF, LAY>(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
-            const double xph = x + h;
-            FOR6 yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
-                                     T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
-                                     T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
-            rhs<SCF, LAY>(P, yy, k3);  // stage 12 reuses k3 (:408)
-            nfcn += 11;
-            FOR6 {
-                k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
-                        T::B[8] * k9[c] + T::B[9] * k10[c] + T::B[10] * k2[c] + T::B[11] * k3[c];
-                k5[c] = y[c] + h * k4[c];
-            }
-            // error estimation (:416-444), n = 6
-            double err = 0.0, err2 = 0.0;
-            FOR6 {
-                const double sk = a.atol + a.rtol * max_d(fabs(y[c]), fabs(k5[c]));
-                const double isk = 1.0 / sk;  // one divide serves both estimates
-                double erri = k4[c] - T::BHH[0] * k1[c] - T::BHH[8] * k9[c] - T::BHH[11] * k3[c];
-                double sqr = erri * isk;
-                err2 += sqr * sqr;
-                erri = T::E5[0] * k1[c] + T::E5[5] * k6[c] + T::E5[6] * k7[c] + T::E5[7] * k8[c] +
-                       T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
-                sqr = erri * isk;
-                err += sqr * sqr;
-            }
-            double deno = err + 0.01 * err2;
-            if (deno <= 0.0) deno = 1.0;
-            err = fabs(h) * err * sqrt(1.0 / (deno * 6.0));
-            // step size factor (:447-452), beta = 0: err^(1/8) as three square
-            // roots (~330 cycles of latency where pow's log/exp pair is ~1000)
-            const double fac11 = sqrt(sqrt(sqrt(err)));
-            const double fac = max_d(facc2, min_d(facc1, fac11 / safe));
-            double hnew = h / fac;
-
-            if (err <= 1.0) {
-                // accepted (:454-637)
-                facold = max_d(err, 1.0e-4);
-                naccpt++;
-                rhs<SCF, LAY>(P, k5, k4);  // FSAL
-                nfcn++;
-                // stiffness test (:463-486); returns -4 on the first detection
-                if (a.nstiff > 0 && (!(naccpt % a.nstiff) || iasti > 0)) {
-                    double stnum = 0.0, stden = 0.0;
-                    FOR6 {
-                        double sqr = k4[c] - k3[c];
-                        stnum += sqr * sqr;
-                        sqr = k5[c] - yy[c];
-                        stden += sqr * sqr;
-                    }
-                    if (stden > 0.0) hlamb = h * sqrt(stnum / stden);
-                    if (hlamb > 6.1) {
-                        nonsti = 0;
-                        iasti++;
-                        code = -4;
-                    } else {
-                        nonsti++;
-                        if (nonsti == 6) iasti = 0;
-                    }
-                }
-                if (code == 0) {
-                    if (DENSE) {
-                        // dense output coefficients (:492-582) and sampling (:584-612)
-                        double r1[6], r2[6], r3[6], r4[6], r5[6], r6[6], r7[6], r8[6];
-                        FOR6 {
-                            r1[c] = y[c];
-                            const double ydiff = k5[c] - y[c];
-                            r2[c] = ydiff;
-                            const double bspl = h * k1[c] - ydiff;
-                            r3[c] = bspl;
-                            r4[c] = ydiff - h * k4[c] - bspl;
-                            r5[c] = T::D[0][0] * k1[c] + T::D[0][5] * k6[c] + T::D[0][6] * k7[c] +
-                                    T::D[0][7] * k8[c] + T::D[0][8] * k9[c] + T::D[0][9] * k10[c] +
-                                    T::D[0][10] * k2[c] + T::D[0][11] * k3[c];
-                            r6[c] = T::D[1][0] * k1[c] + T::D[1][5] * k6[c] + T::D[1][6] * k7[c] +
-                                    T::D[1][7] * k8[c] + T::D[1][8] * k9[c] + T::D[1][9] * k10[c] +
-                                    T::D[1][10] * k2[c] + T::D[1][11] * k3[c];
-                            r7[c] = T::D[2][0] * k1[c] + T::D[2][5] * k6[c] + T::D[2][6] * k7[c] +
-                                    T::D[2][7] * k8[c] + T::D[2][8] * k9[c] + T::D[2][9] * k10[c] +
-                                    T::D[2][10] * k2[c] + T::D[2][11] * k3[c];
-                            r8[c] = T::D[3][0] * k1[c] + T::D[3][5] * k6[c] + T::D[3][6] * k7[c] +
-                                    T::D[3][7] * k8[c] + T::D[3][8] * k9[c] + T::D[3][9] * k10[c] +
-                                    T::D[3][10] * k2[c] + T::D[3][11] * k3[c];
-                        }
-                        FOR6 yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
-                                                 T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
-                                                 T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
-                                                 T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
-                        rhs<SCF, LAY>(P, yy, k10);
-                        FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
-                                                 T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
-                                                 T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
-                                                 T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
-                        rhs<SCF, LAY>(P, yy, k2);
-                        FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
-                                                 T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
-                                                 T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
-                                                 T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
-                        rhs<SCF, LAY>(P, yy, k3);
-                        nfcn += 3;
-                        FOR6 {
-                            r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
-                                         T::D[0][14] * k2[c] + T::D[0][15] * k3[c]);
-                            r6[c] = h * (r6[c] + T::D[1][12] * k4[c] + T::D[1][13] * k10[c] +
-                                         T::D[1][14] * k2[c] + T::D[1][15] * k3[c]);
-                            r7[c] = h * (r7[c] + T::D[2][12] * k4[c] + T::D[2][13] * k10[c] +
-                                         T::D[2][14] * k2[c] + T::D[2][15] * k3[c]);
-                            r8[c] = h * (r8[c] + T::D[3][12] * k4[c] + T::D[3][13] * k10[c] +
-                                         T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
-                        }
-                        const double x0 = x, x1 = x + h;
-                        const int64_t so = a.n_out * a.N;
-                        while (oidx < a.n_out) {
-                            const double to = a.t_out[oidx];
-                            if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
-                                const double s = (to - x) / h, s1 = 1.0 - s;
-                                FOR6 a.w_out[c * so + oidx * a.N + orb] =
-                                    r1[c] + s * (r2[c] + s1 * (r3[c] + s * (r4[c] + s1 * (r5[c] +
-                                    s * (r6[c] + s1 * (r7[c] + s * r8[c]))))));
-                                oidx++;
-                            } else {
-                                break;
-                            }
-                        }
-                    }
-                    FOR6 {
-                        k1[c] = k4[c];
-                        y[c] = k5[c];
-                    }
-                    x = xph;
-                    if (last) {
-                        code = 1;
-                    } else {
-                        if (fabs(hnew) > hmax) hnew = posneg * hmax;
-                        if (reject) hnew = posneg * min_d(fabs(hnew), fabs(h));
-                        reject = 0;
-                    }
-                }
-            } else {
-                // rejected (:638-645)
-                hnew = h / min_d(facc1, fac11 / safe);
-                reject = 1;
-                if (naccpt >= 1) nrejct++;
-            }
-            if (code == 0) h = hnew;
-        }
+        const int code = dop_attempt<SCF, DENSE, LAY, 6>(P, a, 0, s, y, k1);
         if (code != 0) {
-            FOR6 a.w_final[c * a.N + orb] = y[c];
-            if (a.status) a.status[orb] = code;
-            if (a.stats) {
-                a.stats[4 * orb + 0] = nstep;
-                a.stats[4 * orb + 1] = naccpt;
-                a.stats[4 * orb + 2] = nrejct;
-                a.stats[4 * orb + 3] = nfcn;
-            }
+            FOR6 a.w_final[c * a.N + s.orb] = y[c];
+            finish_stats(a, s, code);
             active = false;
         }
     }
@@ -406,6 +573,9 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     a.stats = stats;
     a.order = order;
     a.queue = queue;
+    // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end (same results)
+    const char *wide_env = getenv("GAMD_DOP853_WIDE");
+    a.wide = !(wide_env && wide_env[0] == '0');
     int rc;
     const bool scf = pot->dev.has_scf;
     const bool mw = pot->dev.layout == LAYOUT_MW;

@@ -13,7 +13,7 @@ from . import _device as dev
 from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
-           "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "dop853",
+           "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
            "integrate_hamiltonian"]
 
 
@@ -124,6 +124,20 @@ def leapfrog(potential, w0, dt, n_steps, save_stride=1, save=True, n_steps_per_o
     return wout, wfin
 
 
+def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
+    """Ruth4 on device (ruth4.pyx:24-106).  w0: (6, N) float64.
+    Returns (w_out (6, n_steps//save_stride + 1, N) or None, w_final (6, N))."""
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    N = w0.shape[1]
+    wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, N)) if save else None
+    wfin = dev.empty((6, N))
+    _lib.check(lib.gamd_ruth4(potential.c_handle, w0.data_ptr(), N, float(t0), float(dt),
+                              int(n_steps), int(save_stride), dev.ptr(wout), wfin.data_ptr(),
+                              dev.stream_ptr()), "gamd_ruth4")
+    return wout, wfin
+
+
 def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0, hmax=0.0,
            t_out=None, order=None, block=0):
     """Per-orbit adaptive DOP853 on device.  w0 (6, N); t0, t1, h0 per orbit
@@ -202,6 +216,12 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
         dt = t[1] - t[0]
         wout, wfin = leapfrog(pot, arr, dt, len(t) - 1, save_stride=1, save=bool(save_all),
                               t0=t[0])
+        w = dev.to_host(wout) if save_all else dev.to_host(wfin)
+    elif Integrator is Ruth4Integrator:
+        if len(t) < 2:
+            raise ValueError("need at least two times")
+        wout, wfin = ruth4(pot, arr, t[1] - t[0], len(t) - 1, save_stride=1,
+                           save=bool(save_all), t0=t[0])
         w = dev.to_host(wout) if save_all else dev.to_host(wfin)
     elif Integrator is DOPRI853Integrator:
         res = dop853(pot, arr, t[0], t[-1], t[1] - t[0],

@@ -210,10 +210,13 @@ class SCFSurrogate:
                    "gamd_mlp_forward")
         return phi, grad
 
-    def leapfrog(self, w0, dt, n_steps, tol=1e-5, progress=None):
+    def leapfrog(self, w0, dt, n_steps, tol=1e-5, progress=None, check_every=100):
         """Leapfrog of orbits w0 (6, N) with surrogate forces and per-orbit
         exact-SCF fallback once |dE/E| of the surrogate energy exceeds tol
-        (tol < 0: every orbit on the exact force from the start).
+        (tol < 0: every orbit on the exact force from the start).  Every
+        check_every steps, once every orbit has fallen back, the remaining
+        steps run in the persistent exact leapfrog kernel (gamd_leapfrog)
+        from the synchronised state.
         Returns dict(w_final (6,N) device, fallback_step (N,) device int32)."""
         t = dev.torch()
         w = dev.to_dev(w0).clone()
@@ -233,5 +236,11 @@ class SCFSurrogate:
                 _lib.check(rc, "surrogate step")
             if progress and j % 1000 == 0:
                 progress(j)
+            if j < n_steps and check_every and j % check_every == 0 and \
+                    not bool((fb < 0).any().item()):
+                from .integrate import leapfrog as exact_leapfrog
+                ws = t.cat([w[:3], vsync], dim=0).contiguous()
+                _, wf = exact_leapfrog(self.pot, ws, dt, n_steps - j, save=False)
+                return {"w_final": wf, "fallback_step": fb}
         out = t.cat([w[:3], vsync], dim=0) if n_steps > 0 else w
         return {"w_final": out, "fallback_step": fb}

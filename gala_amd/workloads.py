@@ -117,12 +117,29 @@ def mockstream_progenitor():
     return w0, 1e5, 0.01, -1.0, 4000
 
 
-def cost_order(w0, T, gradient_fn):
+def cost_order(w0, T, gradient_fn, spread=64, n_spread=2048):
     """Longest-first queue order for the DOP853 work queue: the estimated
-    number of dynamical times T_i * sqrt(|g| / r) per orbit, descending."""
+    number of dynamical times T_i * sqrt(|g| / r) per orbit, descending.
+
+    The n_spread most expensive orbits go to every spread-th slot instead of
+    the head of the queue: the persistent kernel's waves first claim 64
+    consecutive slots each (2048 waves at 256-thread blocks, two per CU), so
+    each wave then starts with one of them among cheaper orbits, ends up
+    holding it alone, and finishes it in the kernel's 16-lane wide tail."""
     q = w0[:3]
     r = np.sqrt(np.sum(q * q, axis=0))
     g = gradient_fn(q)
     om = np.sqrt(np.sqrt(np.sum(g * g, axis=0)) / r)
     cost = np.abs(T) * om
-    return np.argsort(-cost, kind="stable").astype(np.int32)
+    base = np.argsort(-cost, kind="stable").astype(np.int32)
+    N = base.shape[0]
+    k = min(n_spread, N // spread) if spread > 1 else 0
+    if k <= 0:
+        return base
+    slots = np.arange(k) * spread
+    rest = np.ones(N, dtype=bool)
+    rest[slots] = False
+    out = np.empty_like(base)
+    out[slots] = base[:k]
+    out[rest] = base[k:]
+    return out

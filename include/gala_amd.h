@@ -108,6 +108,16 @@ int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t N, double 
                   int64_t n_steps, const int64_t *n_steps_per_orbit, int64_t save_stride,
                   double *w_out, double *w_final, void *stream);
 
+/* ---- Ruth4 ------------------------------------------------------------- */
+/* Replaces ruth4_integrate_hamiltonian (ruth4.pyx:24-106): fixed dt, per step
+ * four kick-drift substeps with Ruth's coefficients (:67-80).  Arguments and
+ * the saved layout as gamd_leapfrog (no per-orbit step counts); w_final (6,N)
+ * is the state after n_steps steps.  The first substep's kick has d_0 = 0 and
+ * is skipped (identical results wherever the gradient is finite). */
+int gamd_ruth4(const gamd_potential *pot, const double *w0, size_t N, double t0, double dt,
+               int64_t n_steps, int64_t save_stride, double *w_out, double *w_final,
+               void *stream);
+
 /* ---- DOP853 ------------------------------------------------------------- */
 /* Replaces dop853_integrate_hamiltonian / dop853_helper / dop853 /
  * Fwrapper_T (dop853.pyx:90-250, dop853.cpp:89-866, 979-988).  Each orbit is

@@ -14,6 +14,7 @@
  *   composite      potential/potential/src/cpotential.cpp:80-189, 214-287,
  *                  290-314, 346-371, 418-442
  *   leapfrog       integrate/cyintegrators/leapfrog.pyx:24-121, 126-257
+ *   Ruth4          integrate/cyintegrators/ruth4.pyx:24-106
  *   DOP853         integrate/cyintegrators/dopri/dop853.cpp:18-650, 653-866,
  *                  869-904; dop853.pyx:90-250; hamiltonian/src/chamiltonian.cpp:38-57
  *   mock stream    dynamics/mockstream/mockstream.pyx:442-620,
@@ -507,6 +508,55 @@ void orc_leapfrog(const orc_pot *P, const double *w0, size_t N, const double *t,
             for (int k = 0; k < 3; k++) {
                 out[k * N + i] = x[k];
                 out[(3 + k) * N + i] = ntimes > 1 ? v[k] : w0[(3 + k) * N + i];
+            }
+    }
+}
+
+/* Ruth4 (ruth4.pyx:24-106): per step, four substeps j of kick by d_j then
+ * drift by c_j, each kick with the gradient at the current position (the
+ * first, d_0 = 0, leaves v unchanged and is still evaluated, as c_ruth4_step
+ * does).  Coefficients from 2^(1/3) (ruth4.pyx:67-80). */
+void orc_ruth4(const orc_pot *P, const double *w0, size_t N, const double *t, size_t ntimes,
+               int save_all, double *out) {
+    double dt = t[1] - t[0];
+    double two_13 = pow(2.0, 1.0 / 3.0);
+    double cs[4], ds[4];
+    cs[0] = 1.0 / (2.0 * (2.0 - two_13));
+    cs[1] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cs[2] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cs[3] = 1.0 / (2.0 * (2.0 - two_13));
+    ds[0] = 0.0;
+    ds[1] = 1.0 / (2.0 - two_13);
+    ds[2] = -two_13 / (2.0 - two_13);
+    ds[3] = 1.0 / (2.0 - two_13);
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        size_t i = (size_t)ii;
+        double x[3], v[3], g[3];
+        for (int k = 0; k < 3; k++) {
+            x[k] = w0[k * N + i];
+            v[k] = w0[(3 + k) * N + i];
+        }
+        if (save_all)
+            for (int k = 0; k < 6; k++) out[(k * ntimes + 0) * N + i] = w0[k * N + i];
+        for (size_t j = 1; j < ntimes; j++) {
+            for (int s = 0; s < 4; s++) {
+                grad1(P, x, g);
+                for (int k = 0; k < 3; k++) {
+                    v[k] = v[k] - ds[s] * g[k] * dt;
+                    x[k] = x[k] + cs[s] * v[k] * dt;
+                }
+            }
+            if (save_all)
+                for (int k = 0; k < 3; k++) {
+                    out[(k * ntimes + j) * N + i] = x[k];
+                    out[((3 + k) * ntimes + j) * N + i] = v[k];
+                }
+        }
+        if (!save_all)
+            for (int k = 0; k < 3; k++) {
+                out[k * N + i] = x[k];
+                out[(3 + k) * N + i] = v[k];
             }
     }
 }

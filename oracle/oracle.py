@@ -61,6 +61,7 @@ def lib(fast=False):
         L.orc_d2_dr2.restype = C.c_double
         L.orc_d2_dr2.argtypes = [P, C.c_double, dp]
         L.orc_leapfrog.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_int, dp]
+        L.orc_ruth4.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_int, dp]
         L.orc_dop853.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_double, C.c_double,
                                  C.c_long, C.c_long, C.c_double, C.c_int, C.c_int, dp, ip, lp]
         L.orc_dop853_orbits.argtypes = [P, dp, C.c_size_t, dp, dp, dp, C.c_double, C.c_double,
@@ -238,6 +239,17 @@ def leapfrog(components, w0, t, save_all=True, fast=False):
     N = w0.shape[1]
     out = np.zeros((6, len(t), N) if save_all else (6, N))
     lib(fast).orc_leapfrog(C.byref(P), w0, N, t, len(t), int(save_all), out.reshape(-1))
+    return out
+
+
+def ruth4(components, w0, t, save_all=True, fast=False):
+    """Ruth4 (ruth4.pyx:24-106): w0 (6,N), t (ntimes,) -> (6,ntimes,N) or (6,N)"""
+    P = make_pot(components)
+    w0 = _c(w0).reshape(6, -1)
+    t = _c(t)
+    N = w0.shape[1]
+    out = np.zeros((6, len(t), N) if save_all else (6, N))
+    lib(fast).orc_ruth4(C.byref(P), w0, N, t, len(t), int(save_all), out.reshape(-1))
     return out
 
 

@@ -104,3 +104,32 @@ def test_queue_order_invariance():
     b = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, order=order, block=128)
     assert np.array_equal(dev.to_host(a["w_final"]), dev.to_host(b["w_final"]))
     assert np.array_equal(dev.to_host(a["stats"]), dev.to_host(b["stats"]))
+
+
+@pytest.mark.parametrize("layout", ["milky_way", "generic"])
+def test_wide_tail_matches_one_lane_per_orbit(layout, monkeypatch):
+    """The last orbits of each wave finish spread over 16-lane rows; that
+    changes the lane layout only, never a bit of the result (final states,
+    status codes and step statistics), including the stiffness test, which
+    nstiff=3 runs every few steps."""
+    from gala_amd.integrate import dop853
+    from gala_amd import _device as dev
+    from gala_amd.potential import HernquistPotential, PlummerPotential
+    orc = mw_oracle()
+    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=3000, seed=21)
+    T = T * 0.05
+    if layout == "milky_way":
+        pot = mw_product()
+    else:
+        pot = HernquistPotential(m=1e11, c=2.0) + PlummerPotential(m=1e10, b=1.0,
+                                                                   origin=[1.0, 0.5, -0.2])
+    out = {}
+    for wide in ("0", "1"):
+        monkeypatch.setenv("GAMD_DOP853_WIDE", wide)
+        for nstiff in (0, 3):
+            r = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, nstiff=nstiff)
+            out[wide, nstiff] = [dev.to_host(r[k]) for k in ("w_final", "status", "stats")]
+    for nstiff in (0, 3):
+        for a, b in zip(out["0", nstiff], out["1", nstiff]):
+            assert np.array_equal(a, b)
+    assert np.all(out["1", 0][1] == 1)

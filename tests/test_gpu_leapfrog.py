@@ -112,3 +112,39 @@ def test_uneven_lengths():
         ref = O.leapfrog(orc, w0[:, sel], np.arange(L + 1) * 1.0, save_all=False) if L > 0 \
             else w0[:, sel]
         assert np.max(state_rel_err(w[:, sel], ref)) < 1e-11
+
+
+def test_ruth4_composite_parity():
+    """gamd_ruth4 against the oracle's Ruth4 (ruth4.pyx:24-106) on the
+    1024-orbit composite workload, 1e-10 relative or 100x the orbit's own
+    one-ulp sensitivity; the saved grid's last step equals the final state."""
+    from gala_amd.integrate import ruth4
+    from gala_amd import _device as dev
+    orc = mw_oracle()
+    w0, dt, _ = workloads.composite_leapfrog(vcirc_oracle(orc), n=1024, seed=0)
+    nsteps = 1000
+    t = np.arange(nsteps + 1) * dt
+    ref = O.ruth4(orc, w0, t, save_all=False)
+    pert = w0.copy()
+    pert[0] = np.nextafter(pert[0], np.inf)
+    tol = np.maximum(1e-10, 100.0 * state_rel_err(O.ruth4(orc, pert, t, save_all=False), ref))
+    wout, wfin = ruth4(mw_product(), w0, dt, nsteps, save_stride=50)
+    w = dev.to_host(wfin)
+    err = state_rel_err(w, ref)
+    print(f"ruth4 max rel err {err.max():.3e}; {np.sum(tol > 1e-10)} orbit(s) gated")
+    assert np.all(err <= tol)
+    wo = dev.to_host(wout)
+    assert np.array_equal(wo[:, 0], w0)
+    assert np.array_equal(wo[:, -1], w)
+
+
+def test_ruth4_integrate_orbit_save_all():
+    from gala_amd.potential import Hamiltonian
+    orc = mw_oracle()
+    rng = np.random.default_rng(8)
+    w0 = np.vstack([rng.uniform(5, 15, (3, 16)), rng.normal(0, 0.15, (3, 16))])
+    t = np.arange(201) * 0.5
+    ref = O.ruth4(orc, w0, t, save_all=True)
+    orbit = Hamiltonian(mw_product()).integrate_orbit(w0, t=t, Integrator="ruth4")
+    assert orbit.xyz.shape == (3, 201, 16)
+    assert np.max(state_rel_err(orbit.w(), ref)) < 1e-11
