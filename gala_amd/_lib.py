@@ -58,6 +58,8 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _vp, _int, _vp]),
     "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _d, _vp,
                                         _vp]),
+    "gamd_mockstream_dop853": (_int, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _d, _d, _d, _d,
+                                      _i64, _vp, _vp, _vp, _vp, _vp]),
     "gamd_mlp_param_count": (_i64, []),
     "gamd_mlp_bf16_count": (_i64, []),
     "gamd_mlp_workspace_bytes": (_i64, [_i64]),

@@ -624,6 +624,32 @@ __device__ __forceinline__ void pot_grad(const DevPot &P, double x, double y, do
     }
 }
 
+// Gradient of a body's potential PP at q, centred on the body's position xp:
+// c_gradient of the potential with every component's origin set to xp
+// (c_nbody_acceleration / c_nbody_gradient_symplectic force do_shift_rotate
+// = 1, cpotential.cpp:389-442): q' = R (q - xp), gradient rotated back with
+// R^T, summed over components into a zeroed f2 that the caller adds.
+template <bool WITH_SCF>
+__device__ __forceinline__ void body_grad(const DevPot &PP, double x, double y, double z,
+                                          double px, double py, double pz, double &fx,
+                                          double &fy, double &fz) {
+    fx = 0.0;
+    fy = 0.0;
+    fz = 0.0;
+    for (int i = 0; i < PP.n; i++) {
+        const DevComp &c = PP.c[i];
+        const double tx = x - px, ty = y - py, tz = z - pz;
+        const double xt = c.R[0] * tx + c.R[1] * ty + c.R[2] * tz;
+        const double yt = c.R[3] * tx + c.R[4] * ty + c.R[5] * tz;
+        const double zt = c.R[6] * tx + c.R[7] * ty + c.R[8] * tz;
+        double ux = 0.0, uy = 0.0, uz = 0.0;
+        comp_grad<WITH_SCF>(c, xt, yt, zt, ux, uy, uz);
+        fx += c.R[0] * ux + c.R[3] * uy + c.R[6] * uz;
+        fy += c.R[1] * ux + c.R[4] * uy + c.R[7] * uz;
+        fz += c.R[2] * ux + c.R[5] * uy + c.R[8] * uz;
+    }
+}
+
 // The Milky-Way layout (nucleus and bulge Hernquist, Miyamoto-Nagai disk, NFW
 // halo, gala's MilkyWayPotential order): the same formulas as the per-
 // component functions above, one shared |q|, contributions accumulated in

@@ -20,6 +20,7 @@
 //     every lane of the wave, which every wave reaches.
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "dop853_tableau.h"
 #include "gamd_host.hpp"
@@ -47,13 +48,14 @@ struct DopArgs {
     const int32_t *order;
     unsigned long long *queue;
     int wide;              // finish the last orbits of a wave in 16-lane rows
+    double uround;         // the "step size too small" threshold (dop853.cpp:349-354)
 };
 
 #define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
 
 // f = (p, -grad Phi(q))
 template <bool SCF, int LAY>
-__device__ __forceinline__ void rhs(const DevPot &P, const double *y, double *f) {
+__device__ __forceinline__ void orbit_rhs(const DevPot &P, const double *y, double *f) {
     double gx, gy, gz;
     pot_grad_l<SCF, LAY>(P, y[0], y[1], y[2], gx, gy, gz);
     f[0] = y[3];
@@ -88,7 +90,7 @@ __device__ double hinit(const DevPot &P, double x, const double *y, double posne
     h = sign_d(h, posneg);
     double yy[6], f1[6];
     FOR6 yy[c] = y[c] + h * f0[c];
-    rhs<SCF, LAY>(P, yy, f1);
+    orbit_rhs<SCF, LAY>(P, yy, f1);
     double der2 = 0.0;
     FOR6 {
         const double sk = atol + rtol * fabs(y[c]);
@@ -137,32 +139,124 @@ __device__ __forceinline__ double row_shl3(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// f = (p, -grad Phi(q)) for the lane's components
-template <bool SCF, int LAY>
-__device__ __forceinline__ void rhs_n(const DevPot &P, int, const double (&y)[6], double (&f)[6]) {
-    rhs<SCF, LAY>(P, y, f);
-}
-template <bool SCF, int LAY>
-__device__ __forceinline__ void rhs_n(const DevPot &P, int comp, const double (&y)[1],
-                                      double (&f)[1]) {
-    const double qx = row_bcast<0>(y[0]), qy = row_bcast<1>(y[0]), qz = row_bcast<2>(y[0]);
-    const double vp = row_shl3(y[0]);  // the velocity partner of components 0..2
-    double gx, gy, gz;
-    pot_grad_l<SCF, LAY>(P, qx, qy, qz, gx, gy, gz);
-    const double g = comp == 3 ? gx : (comp == 4 ? gy : gz);
-    f[0] = comp < 3 ? vp : -g;
+// DPP quad_perm [B,B,B,B]: every lane of a 4-lane quad reads lane B of the quad
+template <int B>
+__device__ __forceinline__ double quad_bcast(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x55 * B, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x55 * B, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-// the six per-component values of an orbit, in component order
-__device__ __forceinline__ void all6(const double (&v)[6], double (&o)[6]) { FOR6 o[c] = v[c]; }
-__device__ __forceinline__ void all6(const double (&v)[1], double (&o)[6]) {
-    o[0] = row_bcast<0>(v[0]);
-    o[1] = row_bcast<1>(v[0]);
-    o[2] = row_bcast<2>(v[0]);
-    o[3] = row_bcast<3>(v[0]);
-    o[4] = row_bcast<4>(v[0]);
-    o[5] = row_bcast<5>(v[0]);
-}
+// Lane layouts of one DOP853 system (the policy of dop_attempt):
+//   rhs   the lane's share of f(y);
+//   sums  sa += a_i^2, sb += b_i^2 over the system's n components in the
+//         order of its state vector (the error norm and stiffness sums);
+//   n     the system's number of components.
+
+// one orbit per lane: f = (p, -grad Phi(q))
+template <bool SCF, int LAY>
+struct OneLane {
+    static constexpr int NC = 6;
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
+        orbit_rhs<SCF, LAY>(P, y, f);
+    }
+    __device__ __forceinline__ void sums(const double (&a)[6], const double (&b)[6], double &sa,
+                                         double &sb) const {
+        FOR6 {
+            sa += a[c] * a[c];
+            sb += b[c] * b[c];
+        }
+    }
+    __device__ __forceinline__ double n() const { return 6.0; }
+};
+
+// one orbit per 16-lane row, lane c holding component comp
+template <bool SCF, int LAY>
+struct WideRow {
+    static constexpr int NC = 1;
+    int comp;
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[1], double (&f)[1]) const {
+        const double qx = row_bcast<0>(y[0]), qy = row_bcast<1>(y[0]), qz = row_bcast<2>(y[0]);
+        const double vp = row_shl3(y[0]);  // the velocity partner of components 0..2
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(P, qx, qy, qz, gx, gy, gz);
+        const double g = comp == 3 ? gx : (comp == 4 ? gy : gz);
+        f[0] = comp < 3 ? vp : -g;
+    }
+    __device__ __forceinline__ static void all6(const double (&v)[1], double (&o)[6]) {
+        o[0] = row_bcast<0>(v[0]);
+        o[1] = row_bcast<1>(v[0]);
+        o[2] = row_bcast<2>(v[0]);
+        o[3] = row_bcast<3>(v[0]);
+        o[4] = row_bcast<4>(v[0]);
+        o[5] = row_bcast<5>(v[0]);
+    }
+    __device__ __forceinline__ void sums(const double (&a)[1], const double (&b)[1], double &sa,
+                                         double &sb) const {
+        double qa[6], qb[6];
+        all6(a, qa);
+        all6(b, qb);
+        FOR6 {
+            sa += qa[c] * qa[c];
+            sb += qb[c] * qb[c];
+        }
+    }
+    __device__ __forceinline__ double n() const { return 6.0; }
+};
+
+// one mock-stream release system per 4-lane quad (Fwrapper_direct_nbody,
+// dop853.cpp:990-1006): lane b holds body b's six components, body 0 the
+// progenitor, bodies 1..nb-1 massless particles that also feel the
+// progenitor's potential PP centred on body 0 (c_nbody_acceleration,
+// cpotential.cpp:389-415).  Lanes b >= nb mirror body 0 and are left out of
+// every sum.  The state vector is body-major, as the reference's (nb, 6) rows.
+template <bool SCF, int LAY>
+struct QuadBodies {
+    static constexpr int NC = 6;
+    int body, nb;
+    const DevPot *PP;  // nullptr: the progenitor exerts no force
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
+        const double px = quad_bcast<0>(y[0]), py = quad_bcast<0>(y[1]), pz = quad_bcast<0>(y[2]);
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(P, y[0], y[1], y[2], gx, gy, gz);
+        if (PP && body >= 1 && body < nb) {
+            double fx, fy, fz;
+            body_grad<SCF>(*PP, y[0], y[1], y[2], px, py, pz, fx, fy, fz);
+            gx += fx;
+            gy += fy;
+            gz += fz;
+        }
+        f[0] = y[3];
+        f[1] = y[4];
+        f[2] = y[5];
+        f[3] = -gx;
+        f[4] = -gy;
+        f[5] = -gz;
+    }
+    template <int B>
+    __device__ __forceinline__ void sums_body(const double (&a)[6], const double (&b)[6],
+                                              double &sa, double &sb) const {
+        double qa[6], qb[6];
+        FOR6 {
+            qa[c] = quad_bcast<B>(a[c]);
+            qb[c] = quad_bcast<B>(b[c]);
+        }
+        if (B < nb) {
+            FOR6 {
+                sa += qa[c] * qa[c];
+                sb += qb[c] * qb[c];
+            }
+        }
+    }
+    __device__ __forceinline__ void sums(const double (&a)[6], const double (&b)[6], double &sa,
+                                         double &sb) const {
+        sums_body<0>(a, b, sa, sb);
+        sums_body<1>(a, b, sa, sb);
+        sums_body<2>(a, b, sa, sb);
+        sums_body<3>(a, b, sa, sb);
+    }
+    __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
+};
 
 // per-orbit scalars of the integration (wave-uniform within a wide row)
 struct DopState {
@@ -176,51 +270,52 @@ struct DopState {
 
 // One step attempt (dopcor basic step, dop853.cpp:339-648) for the lane's NC
 // components.  Returns 0 to go on, else the orbit's final idid.
-template <bool SCF, bool DENSE, int LAY, int NC>
-__device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, int comp, DopState &s,
-                                           double (&y)[NC], double (&k1)[NC]) {
+template <bool SCF, bool DENSE, int LAY, class Sys>
+__device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, const Sys &sys,
+                                           DopState &s, double (&y)[Sys::NC],
+                                           double (&k1)[Sys::NC]) {
+    constexpr int NC = Sys::NC;
     static_assert(NC == 6 || !DENSE, "dense output runs with one lane per orbit");
-    const double uround = 2.220446049250313e-16;  // np.finfo(float).eps (dop853.pyx:165)
     const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
     if (s.nstep > a.nmax) return -2;
-    if (0.1 * fabs(s.h) <= fabs(s.x) * uround) return -3;
+    if (0.1 * fabs(s.h) <= fabs(s.x) * a.uround) return -3;
     const bool last = (s.x + 1.01 * s.h - s.xend) * s.posneg > 0.0;
     const double h = last ? s.xend - s.x : s.h;
     s.h = h;
     s.nstep++;
     double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], k7[NC], k8[NC], k9[NC], k10[NC], yy[NC];
     FORN yy[c] = y[c] + h * T::A[1][0] * k1[c];
-    rhs_n<SCF, LAY>(P, comp, yy, k2);
+    sys.rhs(P, yy, k2);
     FORN yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k3);
+    sys.rhs(P, yy, k3);
     FORN yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k4);
+    sys.rhs(P, yy, k4);
     FORN yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k5);
+    sys.rhs(P, yy, k5);
     FORN yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k6);
+    sys.rhs(P, yy, k6);
     FORN yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
                              T::A[6][5] * k6[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k7);
+    sys.rhs(P, yy, k7);
     FORN yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
                              T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k8);
+    sys.rhs(P, yy, k8);
     FORN yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
                              T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k9);
+    sys.rhs(P, yy, k9);
     FORN yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
                              T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
                              T::A[9][8] * k9[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k10);
+    sys.rhs(P, yy, k10);
     FORN yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
                              T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
                              T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
+    sys.rhs(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
     const double xph = s.x + h;
     FORN yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
                              T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
                              T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
-    rhs_n<SCF, LAY>(P, comp, yy, k3);  // stage 12 reuses k3 (:408)
+    sys.rhs(P, yy, k3);  // stage 12 reuses k3 (:408)
     s.nfcn += 11;
     FORN {
         k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
@@ -238,17 +333,11 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, in
                T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
         e1[c] = erri * isk;
     }
-    double q2[6], q1[6];
-    all6(e2, q2);
-    all6(e1, q1);
     double err = 0.0, err2 = 0.0;
-    FOR6 {
-        err2 += q2[c] * q2[c];
-        err += q1[c] * q1[c];
-    }
+    sys.sums(e2, e1, err2, err);
     double deno = err + 0.01 * err2;
     if (deno <= 0.0) deno = 1.0;
-    err = fabs(h) * err * sqrt(1.0 / (deno * 6.0));
+    err = fabs(h) * err * sqrt(1.0 / (deno * sys.n()));
     // step size factor (:447-452), beta = 0: err^(1/8) as three square
     // roots (~330 cycles of latency where pow's log/exp pair is ~1000)
     const double fac11 = sqrt(sqrt(sqrt(err)));
@@ -259,7 +348,7 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, in
         // accepted (:454-637)
         s.facold = max_d(err, 1.0e-4);
         s.naccpt++;
-        rhs_n<SCF, LAY>(P, comp, k5, k4);  // FSAL
+        sys.rhs(P, k5, k4);  // FSAL
         s.nfcn++;
         // stiffness test (:463-486); returns -4 on the first detection
         if (a.nstiff > 0 && (!(s.naccpt % a.nstiff) || s.iasti > 0)) {
@@ -268,14 +357,8 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, in
                 dn[c] = k4[c] - k3[c];
                 dd[c] = k5[c] - yy[c];
             }
-            double qn[6], qd[6];
-            all6(dn, qn);
-            all6(dd, qd);
             double stnum = 0.0, stden = 0.0;
-            FOR6 {
-                stnum += qn[c] * qn[c];
-                stden += qd[c] * qd[c];
-            }
+            sys.sums(dn, dd, stnum, stden);
             if (stden > 0.0) s.hlamb = h * sqrt(stnum / stden);
             if (s.hlamb > 6.1) {
                 s.nonsti = 0;
@@ -314,17 +397,17 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, in
                                          T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
                                          T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
                                          T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
-                rhs<SCF, LAY>(P, yy, k10);
+                orbit_rhs<SCF, LAY>(P, yy, k10);
                 FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
                                          T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
                                          T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
                                          T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
-                rhs<SCF, LAY>(P, yy, k2);
+                orbit_rhs<SCF, LAY>(P, yy, k2);
                 FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
                                          T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
                                          T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
                                          T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
-                rhs<SCF, LAY>(P, yy, k3);
+                orbit_rhs<SCF, LAY>(P, yy, k3);
                 s.nfcn += 3;
                 FOR6 {
                     r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
@@ -429,10 +512,11 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
     s.reject = from_lane(s6.reject, src);
     s.iasti = from_lane(s6.iasti, src);
     s.nonsti = from_lane(s6.nonsti, src);
+    const WideRow<SCF, LAY> row_sys{comp};
     // every row ends: each attempt advances nstep, and nmax bounds it
     while (__any(ract)) {
         if (ract) {
-            const int code = dop_attempt<SCF, false, LAY, 1>(P, a, comp, s, y, k1);
+            const int code = dop_attempt<SCF, false, LAY>(P, a, row_sys, s, y, k1);
             if (code != 0) {
                 if (c < 6) a.w_final[c * a.N + s.orb] = y[0];
                 if (c == 0) finish_stats(a, s, code);
@@ -490,7 +574,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                     s.nfcn = 2;
                     s.oidx = 0;
                     s.reject = s.iasti = s.nonsti = 0;
-                    rhs<SCF, LAY>(P, y, k1);
+                    orbit_rhs<SCF, LAY>(P, y, k1);
                     if (s.h == 0.0)
                         s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
                     active = true;
@@ -510,11 +594,88 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         }
         if (!active) continue;
 
-        const int code = dop_attempt<SCF, DENSE, LAY, 6>(P, a, 0, s, y, k1);
+        const int code = dop_attempt<SCF, DENSE, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
         if (code != 0) {
             FOR6 a.w_final[c * a.N + s.orb] = y[c];
             finish_stats(a, s, code);
             active = false;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mockstream_dop853's release systems (mockstream.pyx:286-303): system i is
+// the progenitor at its release state plus the release's particles, one
+// (1 + k, 6) system integrated by dop853_step (dop853.pyx:29-79) from
+// rel_t[i] to tfinal; one system per 4-lane quad, all systems concurrently.
+struct StreamArgs {
+    const double *prog_w;     // (nrel, 6) progenitor state at each release
+    const double *rel_t;      // (nrel,)
+    const int32_t *nstream;   // (nrel,) particles released (0..3)
+    const int64_t *offset;    // (nrel,) first particle row of each release
+    const double *stream_w0;  // (ntot, 6)
+    int64_t nrel;
+    double tfinal, dt0;
+    double *out;              // (ntot, 6)
+    double *prog_final;       // (nrel, 6)
+    int32_t *status;          // (nrel,)
+    int64_t *stats;           // (nrel, 4) or nullptr
+};
+
+constexpr int STREAM_MAX_PARTICLES = 3;  // a quad holds the progenitor and three particles
+
+template <bool SCF, int LAY>
+__global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict__ Pp,
+                                                       const DevPot *__restrict__ PPp, int has_prog,
+                                                       StreamArgs sa, DopArgs a) {
+    const DevPot &P = *Pp;
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t rel = gid >> 2;  // uniform over the quad
+    const int body = (int)(gid & 3);
+    if (rel >= sa.nrel) return;
+    const int k = sa.nstream[rel];
+    if (k <= 0 || k > STREAM_MAX_PARTICLES) {
+        if (body == 0) sa.status[rel] = k <= 0 ? 0 : GAMD_ERR_UNSUPPORTED;
+        return;
+    }
+    const int nb = 1 + k;
+    const QuadBodies<SCF, LAY> sys{body, nb, has_prog ? PPp : nullptr};
+    double y[6], k1[6];
+    if (body >= 1 && body < nb) {
+        const double *src = sa.stream_w0 + 6 * (sa.offset[rel] + body - 1);
+        FOR6 y[c] = src[c];
+    } else {
+        FOR6 y[c] = sa.prog_w[6 * rel + c];
+    }
+    DopState s;
+    s.orb = rel;
+    s.x = sa.rel_t[rel];
+    s.xend = sa.tfinal;
+    s.h = sa.dt0;
+    s.hmax = fabs(sa.tfinal - s.x);  // hmax 0 -> xend - x (dop853.cpp:787-788), |.| in dopcor
+    s.posneg = sign_d(1.0, s.xend - s.x);
+    s.facold = 1.0e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nfcn = 2;
+    s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = 0;
+    sys.rhs(P, y, k1);
+    int code = 0;
+    // every quad ends: each attempt advances nstep, and nmax bounds it
+    while (code == 0) code = dop_attempt<SCF, false, LAY>(P, a, sys, s, y, k1);
+    if (body >= 1 && body < nb) {
+        double *dst = sa.out + 6 * (sa.offset[rel] + body - 1);
+        FOR6 dst[c] = y[c];
+    }
+    if (body == 0) {
+        FOR6 sa.prog_final[6 * rel + c] = y[c];
+        sa.status[rel] = code;
+        if (sa.stats) {
+            sa.stats[4 * rel + 0] = s.nstep;
+            sa.stats[4 * rel + 1] = s.naccpt;
+            sa.stats[4 * rel + 2] = s.nrejct;
+            sa.stats[4 * rel + 3] = s.nfcn;
         }
     }
 }
@@ -576,6 +737,7 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end (same results)
     const char *wide_env = getenv("GAMD_DOP853_WIDE");
     a.wide = !(wide_env && wide_env[0] == '0');
+    a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
     int rc;
     const bool scf = pot->dev.has_scf;
     const bool mw = pot->dev.layout == LAYOUT_MW;
@@ -590,4 +752,55 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     hipError_t fe = hipFreeAsync(queue, s);
     if (rc == GAMD_OK && fe != hipSuccess) rc = GAMD_ERR_HIP;
     return rc;
+}
+
+extern "C" int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *prog,
+                                      const double *prog_w, size_t nrel, const double *rel_t,
+                                      const int32_t *nstream, const int64_t *offset,
+                                      const double *stream_w0, double tfinal, double dt0,
+                                      double rtol, double atol, int64_t nmax, double *out,
+                                      double *prog_final, int32_t *status, int64_t *stats,
+                                      void *stream) {
+    if (!ext || !prog_w || !rel_t || !nstream || !offset || !prog_final || !status || dt0 == 0.0)
+        return GAMD_ERR_INVALID;
+    if (nmax == 0) nmax = 1000000;  // dop853.cpp:673-674
+    else if (nmax < 0) return GAMD_ERR_INVALID;
+    if (nrel == 0) return GAMD_OK;
+    DopArgs a;
+    memset(&a, 0, sizeof(a));
+    a.rtol = rtol;
+    a.atol = atol;
+    a.nmax = nmax;
+    a.nstiff = 1;          // dop853_step passes nstiff = 1 (dop853.pyx:72)
+    a.uround = 2.3e-16;    // dop853_step passes uround = 0: dop853() picks 2.3e-16 (:733-734)
+    StreamArgs sa;
+    sa.prog_w = prog_w;
+    sa.rel_t = rel_t;
+    sa.nstream = nstream;
+    sa.offset = offset;
+    sa.stream_w0 = stream_w0;
+    sa.nrel = (int64_t)nrel;
+    sa.tfinal = tfinal;
+    sa.dt0 = dt0;
+    sa.out = out;
+    sa.prog_final = prog_final;
+    sa.status = status;
+    sa.stats = stats;
+    const bool scf = ext->dev.has_scf || (prog && prog->dev.has_scf);
+    const bool mw = ext->dev.layout == LAYOUT_MW && !(prog && prog->dev.has_scf);
+    const DevPot *pp = prog ? prog->ddev : ext->ddev;  // not read without a progenitor
+    const int has_prog = prog ? 1 : 0;
+    const int block = 256;
+    const unsigned grid = (unsigned)((4 * (int64_t)nrel + block - 1) / block);
+    hipStream_t s = as_stream(stream);
+    if (scf)
+        hipLaunchKernelGGL((k_dop853_stream<true, LAYOUT_GENERIC>), dim3(grid), dim3(block), 0, s,
+                           ext->ddev, pp, has_prog, sa, a);
+    else if (mw)
+        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_MW>), dim3(grid), dim3(block), 0, s,
+                           ext->ddev, pp, has_prog, sa, a);
+    else
+        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_GENERIC>), dim3(grid), dim3(block), 0, s,
+                           ext->ddev, pp, has_prog, sa, a);
+    return launch_status();
 }

@@ -25,24 +25,16 @@ using namespace gamd;
 
 namespace {
 
-// gradient of the progenitor potential at q, centred on xp (the reference
-// forces do_shift_rotate = 1 with q0 = progenitor position)
+// gradient of the progenitor potential at q, centred on xp, added to g
 template <bool SCF>
 __device__ __forceinline__ void prog_grad(const DevPot &PP, double x, double y, double z,
                                           double px, double py, double pz, double &gx,
                                           double &gy, double &gz) {
-    for (int i = 0; i < PP.n; i++) {
-        const DevComp &c = PP.c[i];
-        const double tx = x - px, ty = y - py, tz = z - pz;
-        const double xt = c.R[0] * tx + c.R[1] * ty + c.R[2] * tz;
-        const double yt = c.R[3] * tx + c.R[4] * ty + c.R[5] * tz;
-        const double zt = c.R[6] * tx + c.R[7] * ty + c.R[8] * tz;
-        double ux = 0.0, uy = 0.0, uz = 0.0;
-        comp_grad<SCF>(c, xt, yt, zt, ux, uy, uz);
-        gx += c.R[0] * ux + c.R[3] * uy + c.R[6] * uz;
-        gy += c.R[1] * ux + c.R[4] * uy + c.R[7] * uz;
-        gz += c.R[2] * ux + c.R[5] * uy + c.R[8] * uz;
-    }
+    double fx, fy, fz;
+    body_grad<SCF>(PP, x, y, z, px, py, pz, fx, fy, fz);
+    gx += fx;
+    gy += fy;
+    gz += fz;
 }
 
 template <bool SCF>

@@ -352,9 +352,9 @@ class MockStreamGenerator:
         if output_every is not None:
             raise NotImplementedError("snapshot output (output_every=) is not supported")
         Integrator = get_integrator(Integrator or DOPRI853Integrator)
-        if Integrator is not LeapfrogIntegrator:
+        if Integrator not in (LeapfrogIntegrator, DOPRI853Integrator):
             raise NotImplementedError("mock streams are generated with Integrator='leapfrog' "
-                                      "only; DOP853 streams are not implemented")
+                                      "or 'dop853'")
         units = self.hamiltonian.units
         pot = self.hamiltonian.potential
         t = parse_time_specification(units, **time_spec)
@@ -362,6 +362,9 @@ class MockStreamGenerator:
             pw0 = prog_w0.w().reshape(6, 1)
         else:
             pw0 = np.asarray(prog_w0, dtype=np.float64).reshape(6, 1)
+        if Integrator is DOPRI853Integrator:
+            return self._run_dop853(pw0, prog_mass, t, release_every, n_particles,
+                                    dict(Integrator_kwargs or {}))
         # progenitor orbit over t (backward if dt < 0), then reversed
         wout, _ = leapfrog(pot, pw0, t[1] - t[0], len(t) - 1, save=True)
         w = dev.to_host(wout)[:, :, 0]
@@ -412,5 +415,94 @@ class MockStreamGenerator:
                             release_time=stream_w0.release_time,
                             lead_trail=stream_w0.lead_trail, frame=self.hamiltonian.frame)
         prog = PhaseSpacePosition(pos=prog_final[:3, None], vel=prog_final[3:, None],
+                                  frame=self.hamiltonian.frame)
+        return stream, prog
+
+    def _run_dop853(self, pw0, prog_mass, t, release_every, n_particles, kw):
+        """The DOP853 path of run() (mockstream_generator.py:229-330 with
+        mockstream_dop853, mockstream.pyx:176-303):
+          1. the progenitor's orbit over t by DOP853 with dense output at t
+             (DirectNBody.integrate_orbit -> direct_nbody_dop853, stiffness
+             test off), reversed when t runs backwards;
+          2. the DF samples particles along it;
+          3. the progenitor again, forward from the earliest time, densely
+             output at the release times (dop853_helper in mockstream_dop853);
+          4. every release's progenitor + particles system by DOP853 to the
+             final time (gamd_mockstream_dop853)."""
+        from .integrate import dop853
+        pot = self.hamiltonian.potential
+        atol = float(kw.get("atol", 1e-10))
+        rtol = float(kw.get("rtol", 1e-10))
+        nmax = int(kw.get("nmax", 0))
+        dt_max = float(kw.get("dt_max", 0.0))
+        nstiff = int(kw.get("nstiff", -1))
+        err_if_fail = int(kw.get("err_if_fail", 1))
+
+        def checked(res, what):
+            st = dev.to_host(res["status"])
+            if err_if_fail == 1 and np.any(st < 0):
+                raise RuntimeError(f"Integration failed with code {int(st[st < 0][0])} ({what})")
+            return res
+
+        res = checked(dop853(pot, pw0, t[0], t[-1], t[1] - t[0], rtol=rtol, atol=atol, nmax=nmax,
+                             nstiff=-1, hmax=dt_max, t_out=t), "progenitor orbit")
+        w = dev.to_host(res["w_out"])[:, :, 0]
+        orbit_t = t
+        if t[1] < t[0]:
+            w = w[:, ::-1]
+            orbit_t = t[::-1]
+        prog_orbit = Orbit(pos=w[:3], vel=w[3:], t=orbit_t, hamiltonian=self.hamiltonian)
+        stream_w0 = self.df.sample(prog_orbit, prog_mass, hamiltonian=self.hamiltonian,
+                                   release_every=release_every, n_particles=n_particles)
+        # release bookkeeping (mockstream_generator.py:272-292)
+        unq_t1s, nstream = np.unique(stream_w0.release_time, return_counts=True)
+        all_nstream = np.zeros(len(orbit_t), dtype=int)
+        for t1, n in zip(unq_t1s, nstream):
+            all_nstream[np.isclose(orbit_t, t1)] = n
+        nstream_idx = np.where(all_nstream != 0)[0]
+        if 0 not in nstream_idx:
+            nstream_idx = np.insert(nstream_idx, 0, 0)
+            unq_t1s = np.insert(unq_t1s, 0, orbit_t[0])
+        time = orbit_t[nstream_idx]
+        ns = all_nstream[nstream_idx].astype(np.int32)
+        if len(time) < 2:
+            raise ValueError("a DOP853 mock stream needs at least two release times")
+        if ns.max() > 3:
+            raise NotImplementedError("at most 3 particles per release time "
+                                      "(n_particles <= 1 per tail)")
+        # the progenitor, forward, at the release times (mockstream.pyx:274-282)
+        res2 = checked(dop853(pot, np.ascontiguousarray(w[:, 0:1]), time[0], time[-1],
+                              time[1] - time[0], rtol=rtol, atol=atol, nmax=nmax,
+                              nstiff=nstiff, hmax=dt_max, t_out=time), "progenitor at releases")
+        prog_w = dev.to_dev(np.ascontiguousarray(dev.to_host(res2["w_out"])[:, :, 0].T))
+        nrel = len(time)
+        offsets = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+        sw = np.ascontiguousarray(stream_w0.w().T)
+        ntot = sw.shape[0]
+        t_ = dev.torch()
+        out = t_.empty((max(ntot, 1), 6), dtype=t_.float64, device="cuda")
+        prog_final = t_.empty((nrel, 6), dtype=t_.float64, device="cuda")
+        status = t_.empty(nrel, dtype=t_.int32, device="cuda")
+        swd = dev.to_dev(sw)
+        rel_d = dev.to_dev(np.ascontiguousarray(unq_t1s, dtype=np.float64))
+        ns_d = dev.to_dev(ns, dtype=t_.int32)
+        off_d = dev.to_dev(offsets, dtype=t_.int64)
+        lib = _lib.load()
+        prog_handle = self.progenitor_potential.c_handle if self.self_gravity else None
+        _lib.check(lib.gamd_mockstream_dop853(pot.c_handle, prog_handle, prog_w.data_ptr(), nrel,
+                                              rel_d.data_ptr(), ns_d.data_ptr(), off_d.data_ptr(),
+                                              swd.data_ptr(), float(orbit_t[-1]),
+                                              float(time[1] - time[0]), rtol, atol, nmax,
+                                              out.data_ptr(), prog_final.data_ptr(),
+                                              status.data_ptr(), None, dev.stream_ptr()),
+                   "gamd_mockstream_dop853")
+        checked({"status": status}, "release systems")
+        raw_stream = dev.to_host(out)[:ntot].T
+        last = int(np.nonzero(ns)[0][-1])
+        pf = dev.to_host(prog_final)[last]
+        stream = MockStream(pos=raw_stream[:3], vel=raw_stream[3:],
+                            release_time=stream_w0.release_time,
+                            lead_trail=stream_w0.lead_trail, frame=self.hamiltonian.frame)
+        prog = PhaseSpacePosition(pos=pf[:3, None], vel=pf[3:, None],
                                   frame=self.hamiltonian.frame)
         return stream, prog

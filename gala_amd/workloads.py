@@ -117,15 +117,17 @@ def mockstream_progenitor():
     return w0, 1e5, 0.01, -1.0, 4000
 
 
-def cost_order(w0, T, gradient_fn, spread=64, n_spread=2048):
+def cost_order(w0, T, gradient_fn, spread=64, per_wave=4, n_spread=2048):
     """Longest-first queue order for the DOP853 work queue: the estimated
     number of dynamical times T_i * sqrt(|g| / r) per orbit, descending.
 
-    The n_spread most expensive orbits go to every spread-th slot instead of
-    the head of the queue: the persistent kernel's waves first claim 64
-    consecutive slots each (2048 waves at 256-thread blocks, two per CU), so
-    each wave then starts with one of them among cheaper orbits, ends up
-    holding it alone, and finishes it in the kernel's 16-lane wide tail."""
+    The n_spread most expensive orbits are dealt per_wave at a time to the
+    first slots of consecutive spread-slot blocks instead of filling the head
+    of the queue: the persistent kernel's waves first claim 64 consecutive
+    slots each, so each of n_spread / per_wave waves starts with per_wave of
+    them among cheaper orbits, holds only those once the queue is drained,
+    and finishes them in the kernel's wide tail (up to four orbits on 16-lane
+    rows).  spread <= 1 or n_spread = 0 keeps the plain longest-first order."""
     q = w0[:3]
     r = np.sqrt(np.sum(q * q, axis=0))
     g = gradient_fn(q)
@@ -133,10 +135,14 @@ def cost_order(w0, T, gradient_fn, spread=64, n_spread=2048):
     cost = np.abs(T) * om
     base = np.argsort(-cost, kind="stable").astype(np.int32)
     N = base.shape[0]
-    k = min(n_spread, N // spread) if spread > 1 else 0
+    if spread <= 1 or per_wave < 1 or per_wave > spread:
+        return base
+    k = min(n_spread, (N // spread) * per_wave)
+    k -= k % per_wave
     if k <= 0:
         return base
-    slots = np.arange(k) * spread
+    blocks = np.arange(k // per_wave) * spread
+    slots = (blocks[:, None] + np.arange(per_wave)[None, :]).ravel()
     rest = np.ones(N, dtype=bool)
     rest[slots] = False
     out = np.empty_like(base)

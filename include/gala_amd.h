@@ -154,6 +154,27 @@ int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *pr
                              const double *w0, const double *rel_t, size_t n, double tfinal,
                              double *out, void *stream);
 
+/* Replaces the release loop of mockstream_dop853 (mockstream.pyx:286-303),
+ * gala's default mock-stream integrator: release i's system is body 0, the
+ * progenitor at prog_w[i] (its state at the release time from the dense
+ * output of the progenitor's own DOP853 orbit, mockstream.pyx:274-282), plus
+ * the nstream[i] particles in rows offset[i].. of stream_w0 (ntot, 6); the
+ * (1 + k, 6) system is integrated with one shared adaptive step from
+ * rel_t[i] to tfinal with the arguments dop853_step passes (dop853.pyx:29-79:
+ * h = dt0, uround 2.3e-16, nstiff 1, hmax |tfinal - t|) and
+ * Fwrapper_direct_nbody forces (dop853.cpp:990-1006; particles feel prog
+ * centred on body 0).  All arrays are device arrays; nstream[i] <= 3
+ * (status GAMD_ERR_UNSUPPORTED otherwise).  out (ntot, 6): particle states
+ * at tfinal; prog_final (nrel, 6): body 0 of each system at tfinal;
+ * status (nrel): gala's idid (0 for releases without particles); stats
+ * (nrel, 4) nullable. */
+int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *prog,
+                           const double *prog_w, size_t nrel, const double *rel_t,
+                           const int32_t *nstream, const int64_t *offset, const double *stream_w0,
+                           double tfinal, double dt0, double rtol, double atol, int64_t nmax,
+                           double *out, double *prog_final, int32_t *status, int64_t *stats,
+                           void *stream);
+
 /* ---- learned SCF-force surrogate --------------------------------------- */
 /* No counterpart in gala: a 4 -> 256 x 4 (SiLU) -> 4 MLP standing in for the
  * SCF force of scf/src/bfe.cpp:114-189, on BF16 matrix cores with FP32

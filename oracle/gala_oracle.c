@@ -17,7 +17,7 @@
  *   Ruth4          integrate/cyintegrators/ruth4.pyx:24-106
  *   DOP853         integrate/cyintegrators/dopri/dop853.cpp:18-650, 653-866,
  *                  869-904; dop853.pyx:90-250; hamiltonian/src/chamiltonian.cpp:38-57
- *   mock stream    dynamics/mockstream/mockstream.pyx:442-620,
+ *   mock stream    dynamics/mockstream/mockstream.pyx:176-303, 442-620,
  *                  df.pyx:61-106, 363-456, _coord.pyx:23-48
  *   SCF            potential/scf/src/bfe.cpp:20-259, bfe_helper.cpp:14-90
  *                  (GSL special functions replaced by textbook recurrences)
@@ -564,6 +564,15 @@ void orc_ruth4(const orc_pot *P, const double *w0, size_t N, const double *t, si
 /* ------------------------------------------------------------------------ */
 /* DOP853 (dop853.cpp)                                                      */
 
+/* One ODE system for the DOP853 core: n components, right-hand side f. */
+typedef struct orc_sys {
+    int n;
+    void (*f)(const struct orc_sys *S, const double *y, double *dy);
+    const orc_pot *P;    /* external potential */
+    int nb;              /* orbits (Fwrapper_T) or bodies (Fwrapper_direct_nbody) */
+    const orc_pot *prog; /* n-body systems: the potential of body 0, or NULL */
+} orc_sys;
+
 /* hamiltonian_gradient_T for the static frame (chamiltonian.cpp:38-57,
  * builtin_frames.cpp:18-24): f = (p, -grad Phi(q)) over a (6,nb) system. */
 static void orc_fcn(const orc_pot *P, int nb, const double *y, double *f) {
@@ -579,6 +588,30 @@ static void orc_fcn(const orc_pot *P, int nb, const double *y, double *f) {
     }
 }
 
+static void sys_orbits(const orc_sys *S, const double *y, double *f) { orc_fcn(S->P, S->nb, y, f); }
+
+static void particle_grad(const orc_pot *ext, const orc_pot *prog, const double *q,
+                          const double *xp, double *g);
+
+/* Fwrapper_direct_nbody (dop853.cpp:990-1006) for bodies in (nb, 6) rows:
+ * f = (p, -grad Phi_ext(q)) per body, then c_nbody_acceleration
+ * (cpotential.cpp:389-415) adds -grad of body 0's potential, centred on
+ * body 0, to every other body (the bodies after 0 are massless). */
+static void sys_nbody(const orc_sys *S, const double *y, double *f) {
+    for (int i = 0; i < S->nb; i++) {
+        const double *q = y + 6 * i;
+        double g[3];
+        if (i == 0 || !S->prog) grad1(S->P, q, g);
+        else particle_grad(S->P, S->prog, q, y, g);
+        f[6 * i + 0] = q[3];
+        f[6 * i + 1] = q[4];
+        f[6 * i + 2] = q[5];
+        f[6 * i + 3] = -g[0];
+        f[6 * i + 4] = -g[1];
+        f[6 * i + 5] = -g[2];
+    }
+}
+
 static double sign_d(double a, double b) { return (b < 0.0) ? -fabs(a) : fabs(a); }
 static double min_d(double a, double b) { return (a < b) ? a : b; }
 static double max_d(double a, double b) { return (a > b) ? a : b; }
@@ -590,11 +623,11 @@ typedef struct {
 /* dopcor, :89-650, specialised to scalar tolerances, no solout, beta=0,
  * meth=1, with gala's dense-output fill at out_t (:584-612).  y (n=6 nb)
  * is advanced in place.  Returns the idid code. */
-static int orc_dopcor(const orc_pot *P, int nb, double x, double *y, double xend, double hmax,
+static int orc_dopcor(const orc_sys *S, double x, double *y, double xend, double hmax,
                       double h, double rtoli, double atoli, long nmax, long nstiff, double uround,
                       const double *out_t, int n_out, double *out_y, double *work,
                       orc_dopstats *st) {
-    const int n = 6 * nb;
+    const int n = S->n;
     const double safe = 0.9, fac1 = 0.333, fac2 = 6.0, beta = 0.0;
     double *k1 = work, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n, *k5 = k4 + n, *k6 = k5 + n,
            *k7 = k6 + n, *k8 = k7 + n, *k9 = k8 + n, *k10 = k9 + n, *yy1 = k10 + n,
@@ -611,7 +644,7 @@ static int orc_dopcor(const orc_pot *P, int nb, double x, double *y, double xend
     double hlamb = 0.0, xold = x, hout = h;
     st->nstep = st->naccpt = st->nrejct = st->nfcn = 0;
 
-    orc_fcn(P, nb, y, k1);
+    S->f(S, y, k1);
     hmax = fabs(hmax);
     st->nfcn += 2; /* :319 (counts the hinit evaluation slot too) */
 
@@ -628,44 +661,44 @@ static int orc_dopcor(const orc_pot *P, int nb, double x, double *y, double xend
 
         /* the twelve stages, :369-408 */
         for (int i = 0; i < n; i++) yy1[i] = y[i] + h * A[1][0] * k1[i];
-        orc_fcn(P, nb, yy1, k2);
+        S->f(S, yy1, k2);
         for (int i = 0; i < n; i++) yy1[i] = y[i] + h * (A[2][0] * k1[i] + A[2][1] * k2[i]);
-        orc_fcn(P, nb, yy1, k3);
+        S->f(S, yy1, k3);
         for (int i = 0; i < n; i++) yy1[i] = y[i] + h * (A[3][0] * k1[i] + A[3][2] * k3[i]);
-        orc_fcn(P, nb, yy1, k4);
+        S->f(S, yy1, k4);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[4][0] * k1[i] + A[4][2] * k3[i] + A[4][3] * k4[i]);
-        orc_fcn(P, nb, yy1, k5);
+        S->f(S, yy1, k5);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[5][0] * k1[i] + A[5][3] * k4[i] + A[5][4] * k5[i]);
-        orc_fcn(P, nb, yy1, k6);
+        S->f(S, yy1, k6);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[6][0] * k1[i] + A[6][3] * k4[i] + A[6][4] * k5[i] + A[6][5] * k6[i]);
-        orc_fcn(P, nb, yy1, k7);
+        S->f(S, yy1, k7);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[7][0] * k1[i] + A[7][3] * k4[i] + A[7][4] * k5[i] +
                                  A[7][5] * k6[i] + A[7][6] * k7[i]);
-        orc_fcn(P, nb, yy1, k8);
+        S->f(S, yy1, k8);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[8][0] * k1[i] + A[8][3] * k4[i] + A[8][4] * k5[i] +
                                  A[8][5] * k6[i] + A[8][6] * k7[i] + A[8][7] * k8[i]);
-        orc_fcn(P, nb, yy1, k9);
+        S->f(S, yy1, k9);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[9][0] * k1[i] + A[9][3] * k4[i] + A[9][4] * k5[i] +
                                  A[9][5] * k6[i] + A[9][6] * k7[i] + A[9][7] * k8[i] +
                                  A[9][8] * k9[i]);
-        orc_fcn(P, nb, yy1, k10);
+        S->f(S, yy1, k10);
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[10][0] * k1[i] + A[10][3] * k4[i] + A[10][4] * k5[i] +
                                  A[10][5] * k6[i] + A[10][6] * k7[i] + A[10][7] * k8[i] +
                                  A[10][8] * k9[i] + A[10][9] * k10[i]);
-        orc_fcn(P, nb, yy1, k2);
+        S->f(S, yy1, k2);
         double xph = x + h;
         for (int i = 0; i < n; i++)
             yy1[i] = y[i] + h * (A[11][0] * k1[i] + A[11][3] * k4[i] + A[11][4] * k5[i] +
                                  A[11][5] * k6[i] + A[11][6] * k7[i] + A[11][7] * k8[i] +
                                  A[11][8] * k9[i] + A[11][9] * k10[i] + A[11][10] * k2[i]);
-        orc_fcn(P, nb, yy1, k3);
+        S->f(S, yy1, k3);
         st->nfcn += 11;
         for (int i = 0; i < n; i++) {
             k4[i] = B[0] * k1[i] + B[5] * k6[i] + B[6] * k7[i] + B[7] * k8[i] + B[8] * k9[i] +
@@ -699,7 +732,7 @@ static int orc_dopcor(const orc_pot *P, int nb, double x, double *y, double xend
             /* step accepted, :454-637 */
             facold = max_d(err, 1.0e-4);
             st->naccpt++;
-            orc_fcn(P, nb, k5, k4);
+            S->f(S, k5, k4);
             st->nfcn++;
 
             /* stiffness detection, :463-486 (the early return fires on the
@@ -744,17 +777,17 @@ static int orc_dopcor(const orc_pot *P, int nb, double x, double *y, double xend
                     yy1[i] = y[i] + h * (A[13][0] * k1[i] + A[13][6] * k7[i] + A[13][7] * k8[i] +
                                          A[13][8] * k9[i] + A[13][9] * k10[i] + A[13][10] * k2[i] +
                                          A[13][11] * k3[i] + A[13][12] * k4[i]);
-                orc_fcn(P, nb, yy1, k10);
+                S->f(S, yy1, k10);
                 for (int i = 0; i < n; i++)
                     yy1[i] = y[i] + h * (A[14][0] * k1[i] + A[14][5] * k6[i] + A[14][6] * k7[i] +
                                          A[14][7] * k8[i] + A[14][10] * k2[i] + A[14][11] * k3[i] +
                                          A[14][12] * k4[i] + A[14][13] * k10[i]);
-                orc_fcn(P, nb, yy1, k2);
+                S->f(S, yy1, k2);
                 for (int i = 0; i < n; i++)
                     yy1[i] = y[i] + h * (A[15][0] * k1[i] + A[15][5] * k6[i] + A[15][6] * k7[i] +
                                          A[15][7] * k8[i] + A[15][8] * k9[i] + A[15][12] * k4[i] +
                                          A[15][13] * k10[i] + A[15][14] * k2[i]);
-                orc_fcn(P, nb, yy1, k3);
+                S->f(S, yy1, k3);
                 st->nfcn += 3;
                 for (int i = 0; i < n; i++)
                     for (int j = 0; j < 4; j++)
@@ -819,7 +852,8 @@ void orc_dop853(const orc_pot *P, const double *w0, size_t N, const double *t, s
         for (int k = 0; k < 6; k++)
             for (int i = 0; i < nb; i++) y[k * nb + i] = w0[k * N + i0 + i];
         orc_dopstats st;
-        int idid = orc_dopcor(P, nb, t[0], y, t[ntimes - 1], hmax, t[1] - t[0], rtol, atol, nmax,
+        orc_sys S = {n, sys_orbits, P, nb, NULL};
+        int idid = orc_dopcor(&S, t[0], y, t[ntimes - 1], hmax, t[1] - t[0], rtol, atol, nmax,
                               nstiff, 2.220446049250313e-16, t, (int)ntimes, oy, work, &st);
         for (int i = 0; i < nb; i++) {
             if (status) status[i0 + i] = idid;
@@ -859,7 +893,8 @@ void orc_dop853_orbits(const orc_pot *P, const double *w0, size_t N, const doubl
         double y[6], work[6 * 19];
         for (int k = 0; k < 6; k++) y[k] = w0[k * N + i];
         orc_dopstats st;
-        int idid = orc_dopcor(P, 1, t0[i], y, t1[i], t1[i] - t0[i], h0[i], rtol, atol, nmax,
+        orc_sys S = {6, sys_orbits, P, 1, NULL};
+        int idid = orc_dopcor(&S, t0[i], y, t1[i], t1[i] - t0[i], h0[i], rtol, atol, nmax,
                               nstiff, 2.220446049250313e-16, NULL, 0, NULL, work, &st);
         for (int k = 0; k < 6; k++) out[k * N + i] = y[k];
         if (status) status[i] = idid;
@@ -980,6 +1015,61 @@ void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const doub
         free(vh);
     }
     free(traj);
+    free(offs);
+}
+
+/* mockstream_dop853's release loop (mockstream.pyx:286-303): for each
+ * release i with nstream[i] > 0, body 0 (the progenitor) starts from
+ * prog_w[i] (6 values, its dense-output state at the release time,
+ * mockstream.pyx:274-282) and the release's particles (the next nstream[i]
+ * rows of stream_w0, (ntot, 6)) follow; the (1 + k, 6) system is integrated
+ * by dop853_step (dop853.pyx:29-79) from stream_t1[i] to tfinal with h = dt0
+ * and the defaults dop853() picks for it (uround 2.3e-16, hmax tfinal - t1,
+ * dop853.cpp:733-792) and nstiff = 1 as dop853_step passes.
+ * out (ntot, 6); prog_final[6]: body 0 after the last release's system;
+ * status[nrel] (0 where nstream[i] = 0), stats (nrel, 4) nullable. */
+void orc_mockstream_dop853(const orc_pot *ext, const orc_pot *prog, const double *prog_w,
+                           size_t nrel, const double *stream_t1, const int *nstream,
+                           const double *stream_w0, double tfinal, double dt0, double atol,
+                           double rtol, long nmax, double *out, double *prog_final, int *status,
+                           long *stats) {
+    if (nmax == 0) nmax = 1000000; /* :673-674 */
+    size_t *offs = (size_t *)malloc(sizeof(size_t) * (nrel + 1));
+    offs[0] = 0;
+    for (size_t i = 0; i < nrel; i++) offs[i + 1] = offs[i] + (size_t)nstream[i];
+    long last_rel = -1;
+    for (size_t i = 0; i < nrel; i++)
+        if (nstream[i] > 0) last_rel = (long)i;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long ii = 0; ii < (long long)nrel; ii++) {
+        size_t i = (size_t)ii;
+        int k = nstream[i];
+        if (status) status[i] = 0;
+        if (k <= 0) continue;
+        int nb = 1 + k, n = 6 * nb;
+        double *y = (double *)malloc(sizeof(double) * n);
+        double *work = (double *)malloc(sizeof(double) * n * 19);
+        for (int c = 0; c < 6; c++) y[c] = prog_w[6 * i + c];
+        for (int j = 0; j < k; j++)
+            for (int c = 0; c < 6; c++) y[6 * (1 + j) + c] = stream_w0[6 * (offs[i] + j) + c];
+        orc_sys S = {n, sys_nbody, ext, nb, prog};
+        orc_dopstats st;
+        int idid = orc_dopcor(&S, stream_t1[i], y, tfinal, tfinal - stream_t1[i], dt0, rtol, atol,
+                              nmax, 1, 2.3e-16, NULL, 0, NULL, work, &st);
+        for (int j = 0; j < k; j++)
+            for (int c = 0; c < 6; c++) out[6 * (offs[i] + j) + c] = y[6 * (1 + j) + c];
+        if ((long)i == last_rel)
+            for (int c = 0; c < 6; c++) prog_final[c] = y[c];
+        if (status) status[i] = idid;
+        if (stats) {
+            stats[4 * i + 0] = st.nstep;
+            stats[4 * i + 1] = st.naccpt;
+            stats[4 * i + 2] = st.nrejct;
+            stats[4 * i + 3] = st.nfcn;
+        }
+        free(y);
+        free(work);
+    }
     free(offs);
 }
 

@@ -68,6 +68,9 @@ def lib(fast=False):
                                         C.c_long, C.c_long, dp, ip, lp]
         L.orc_mockstream_leapfrog.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, C.c_size_t, dp,
                                               ip, dp, C.c_double, dp, dp]
+        L.orc_mockstream_dop853.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, ip, dp,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_long, dp, dp, ip, lp]
         L.orc_fardal_sample.restype = C.c_long
         L.orc_fardal_sample.argtypes = [P, C.c_double, dp, dp, dp, dp, ip, C.c_size_t, C.c_int,
                                         C.c_int, C.c_int, dp, dp, dp, dp]
@@ -97,6 +100,9 @@ def ref_lib():
         L.ref_hessian.argtypes = [P, C.c_double, dp, dp]
         L.ref_d2_dr2.restype = C.c_double
         L.ref_d2_dr2.argtypes = [P, C.c_double, dp]
+        L.ref_mockstream_dop853.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, ip, dp,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_long, dp, dp, ip]
         L.ref_dop853.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_double, C.c_double,
                                  C.c_long, C.c_long, C.c_double, C.c_int, C.c_int, dp, ip]
         _LIBS["ref"] = L
@@ -305,6 +311,34 @@ def mockstream_leapfrog(ext_components, prog_components, full_time, prog_w0, spa
                                       spawn_t, nstream, stream_w0, float(tfinal), prog_final,
                                       stream_final.reshape(-1))
     return prog_final, stream_final
+
+
+def mockstream_dop853(ext_components, prog_components, prog_w, stream_t1, nstream, stream_w0,
+                      tfinal, dt0, atol=1e-10, rtol=1e-10, nmax=0, ref=False):
+    """Release systems of mockstream_dop853 (mockstream.pyx:286-303).
+    prog_w (nrel, 6) progenitor states at the release times; stream_t1 (nrel,);
+    nstream (nrel,) int; stream_w0 (ntot, 6).  Returns (out (ntot, 6),
+    prog_final (6,), status (nrel,), stats (nrel, 4) or None with ref)."""
+    P = make_pot(ext_components)
+    PP = make_pot(prog_components) if prog_components else None
+    prog_w = _c(prog_w).reshape(-1, 6)
+    nrel = prog_w.shape[0]
+    t1 = _c(stream_t1)
+    ns = np.ascontiguousarray(nstream, dtype=np.int32)
+    sw = _c(stream_w0).reshape(-1, 6)
+    out = np.zeros_like(sw)
+    pf = np.zeros(6)
+    status = np.zeros(nrel, dtype=np.int32)
+    if ref:
+        ref_lib().ref_mockstream_dop853(C.byref(P), C.cast(C.byref(PP), C.c_void_p) if PP else None,
+                                        prog_w, nrel, t1, ns, sw, tfinal, dt0, atol, rtol, nmax,
+                                        out, pf, status)
+        return out, pf, status, None
+    stats = np.zeros((nrel, 4), dtype=np.int64)
+    lib().orc_mockstream_dop853(C.byref(P), C.cast(C.byref(PP), C.c_void_p) if PP else None,
+                                prog_w, nrel, t1, ns, sw, tfinal, dt0, atol, rtol, nmax, out, pf,
+                                status, stats)
+    return out, pf, status, stats
 
 
 def fardal_sample(components, G, prog_x, prog_v, prog_t, prog_m, nparticles, z, lead=True,

@@ -203,4 +203,59 @@ void ref_dop853(const orc_pot *P, const double *w0, size_t N, const double *t, s
     destroy(rp);
 }
 
+// mockstream_dop853's release loop (mockstream.pyx:286-303): one call of
+// gala's dop853() per release through the arguments dop853_step passes
+// (dop853.pyx:29-79): Fwrapper_direct_nbody, norbits = 1 + k, nbody = 1,
+// uround/safe/fac/beta/hmax 0 (defaults), h = dt0, meth 0, nstiff 1.
+// prog_w (nrel, 6); stream_w0/out (ntot, 6); status[nrel].
+void ref_mockstream_dop853(const orc_pot *ext, const orc_pot *prog, const double *prog_w,
+                           size_t nrel, const double *stream_t1, const int *nstream,
+                           const double *stream_w0, double tfinal, double dt0, double atol,
+                           double rtol, long nmax, double *out, double *prog_final,
+                           int *status) {
+    RefPot *re = build(ext);
+    CFrameType cf;
+    cf.energy = (energyfunc)static_frame_hamiltonian;
+    cf.gradient = (gradientfunc)static_frame_gradient;
+    cf.hessian = NULL;
+    cf.n_params = 0;
+    cf.parameters = NULL;
+    size_t off = 0;
+    for (size_t i = 0; i < nrel; i++) {
+        const int k = nstream[i];
+        status[i] = 0;
+        if (k <= 0) continue;
+        const unsigned nb = 1 + (unsigned)k, n = 6 * nb;
+        // a fresh progenitor CPotential per release: c_nbody_acceleration
+        // re-points its origin at body 0 on every call
+        RefPot *rp = prog ? build(prog) : nullptr;
+        CPotential *nullp = allocate_cpotential(0);
+        nullp->null = 1;
+        nullp->n_dim = 3;
+        std::vector<CPotential *> pots(nb, nullp);
+        if (rp) {
+            pots[0] = rp->cp;
+        } else {
+            pots[0] = nullp;
+        }
+        std::vector<double> w(n);
+        for (int c = 0; c < 6; c++) w[c] = prog_w[6 * i + c];
+        for (int j = 0; j < k; j++)
+            for (int c = 0; c < 6; c++) w[6 * (1 + j) + c] = stream_w0[6 * (off + j) + c];
+        double rt = rtol, at = atol;
+        int res = dop853(n, (FcnEqDiff)Fwrapper_direct_nbody, re->cp, &cf, nb, 1,
+                         (void *)pots.data(), stream_t1[i], w.data(), tfinal, &rt, &at, 0, NULL,
+                         0, NULL, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dt0, nmax, 0, 1, 0, NULL, 0, NULL,
+                         NULL, 0, NULL);
+        status[i] = res;
+        for (int j = 0; j < k; j++)
+            for (int c = 0; c < 6; c++) out[6 * (off + j) + c] = w[6 * (1 + j) + c];
+        for (int c = 0; c < 6; c++) prog_final[c] = w[c];
+        off += (size_t)k;
+        free_cpotential(nullp);
+        if (rp) destroy(rp);
+    }
+    destroy(re);
+}
+
 }  // extern "C"

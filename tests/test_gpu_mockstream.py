@@ -161,3 +161,108 @@ def test_full_size_stream_against_oracle():
     assert np.median(err) < 1e-10
     assert len(bad) == 0
     np.testing.assert_allclose(prog.w()[:, 0], pf, rtol=1e-9, atol=1e-12)
+
+
+def _dop853_release_inputs(nrel=120, seed=5):
+    """progenitor states at release times (the oracle's dense DOP853 orbit)
+    and two particles per release near it"""
+    prog_w0, prog_mass, b, _, _ = workloads.mockstream_progenitor()
+    orc = mw_oracle()
+    time = -600.0 + 5.0 * np.arange(nrel)
+    out, st, _ = O.dop853(orc, prog_w0[:, None], time, nstiff=-1, nbatch=1)
+    pw = np.ascontiguousarray(out[:, :, 0].T)
+    rng = np.random.default_rng(seed)
+    ns = np.full(nrel, 2, dtype=np.int32)
+    ns[3] = 0
+    ns[10] = 1
+    sw = np.repeat(pw, ns, axis=0) + rng.normal(0, 1.0, (ns.sum(), 6)) * \
+        np.array([0.05, 0.05, 0.05, 0.002, 0.002, 0.002])
+    return orc, prog_mass, b, time, pw, ns, np.ascontiguousarray(sw)
+
+
+def test_dop853_release_kernel_matches_oracle():
+    """gamd_mockstream_dop853 against the oracle (itself bit-for-bit equal to
+    gala's compiled dop853 + Fwrapper_direct_nbody) on identical inputs:
+    status codes equal; particle states within 1e-8 relative, or 100x a
+    particle's own sensitivity to a one-ulp change of its release state."""
+    from gala_amd import _device as dev
+    from gala_amd import _lib
+    from gala_amd.potential import MilkyWayLikePotential, PlummerPotential
+    orc, prog_mass, b, time, pw, ns, sw = _dop853_release_inputs()
+    ext = MilkyWayLikePotential()
+    prog = PlummerPotential(m=prog_mass, b=b)
+    t = dev.torch()
+    nrel, ntot = len(time), sw.shape[0]
+    off = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    out = t.empty((ntot, 6), dtype=t.float64, device="cuda")
+    pf = t.empty((nrel, 6), dtype=t.float64, device="cuda")
+    status = t.empty(nrel, dtype=t.int32, device="cuda")
+    stats = t.empty((nrel, 4), dtype=t.int64, device="cuda")
+    keep = [dev.to_dev(pw), dev.to_dev(time), dev.to_dev(ns, dtype=t.int32),
+            dev.to_dev(off, dtype=t.int64), dev.to_dev(sw)]
+    lib = _lib.load()
+    _lib.check(lib.gamd_mockstream_dop853(ext.c_handle, prog.c_handle, keep[0].data_ptr(), nrel,
+                                          keep[1].data_ptr(), keep[2].data_ptr(),
+                                          keep[3].data_ptr(), keep[4].data_ptr(), 0.0, 5.0,
+                                          1e-10, 1e-10, 0, out.data_ptr(), pf.data_ptr(),
+                                          status.data_ptr(), stats.data_ptr(), dev.stream_ptr()),
+               "gamd_mockstream_dop853")
+    ref_out, ref_pf, ref_st, ref_stats = O.mockstream_dop853(orc, [O.plummer(prog_mass, b)], pw,
+                                                             time, ns, sw, 0.0, 5.0)
+    pert = sw.copy()
+    pert[:, 0] = np.nextafter(pert[:, 0], np.inf)
+    alt = O.mockstream_dop853(orc, [O.plummer(prog_mass, b)], pw, time, ns, pert, 0.0, 5.0)[0]
+    tol = np.maximum(1e-8, 100 * state_rel_err(alt.T, ref_out.T))
+    got = dev.to_host(out)
+    err = state_rel_err(got.T, ref_out.T)
+    st = dev.to_host(status)
+    print(f"dop853 stream kernel: max rel err {err.max():.3e}; {np.sum(tol > 1e-8)} gated; "
+          f"steps equal for {np.mean(dev.to_host(stats)[:, 1] == ref_stats[:, 1]):.3f}")
+    assert np.array_equal(st, ref_st)
+    assert np.all(err <= tol)
+    last = int(np.nonzero(ns)[0][-1])
+    assert state_rel_err(dev.to_host(pf)[last][:, None], ref_pf[:, None])[0] < 1e-8
+
+
+def test_dop853_generator_against_oracle_pipeline():
+    """MockStreamGenerator.run with gala's default DOP853 integrator against
+    the oracle pipeline fed the product's own DF samples: progenitor orbit by
+    DOP853 with dense output, forward again at the release times, then the
+    release systems (mockstream_generator.py:229-330, mockstream.pyx:176-303)."""
+    gen, prog_w0, prog_mass, b, dt, _ = _generator(13)
+    nsteps = 400
+    stream, prog = gen.run(prog_w0, prog_mass, dt=dt, n_steps=nsteps)
+    assert stream.xyz.shape == (3, 2 * (nsteps + 1))
+    orc = mw_oracle()
+    t = np.arange(nsteps + 1) * dt
+    out, st, _ = O.dop853(orc, prog_w0[:, None], t, nstiff=-1, nbatch=1)
+    w = out[:, ::-1, 0]
+    orbit_t = t[::-1]
+    from gala_amd.dynamics import FardalStreamDF, Orbit
+    df = FardalStreamDF(gala_modified=True, random_state=np.random.RandomState(13))
+    ics = df.sample(Orbit(pos=w[:3], vel=w[3:], t=orbit_t, hamiltonian=gen.hamiltonian),
+                    prog_mass, hamiltonian=gen.hamiltonian)
+    rel = np.unique(ics.release_time)
+    ns = np.full(len(rel), 2, dtype=np.int32)
+    fwd, st2, _ = O.dop853(orc, w[:, 0:1], rel, nstiff=-1, nbatch=1)
+    pw = np.ascontiguousarray(fwd[:, :, 0].T)
+    sw = np.ascontiguousarray(ics.w().T)
+    ref_out, ref_pf, ref_st, _ = O.mockstream_dop853(orc, [O.plummer(prog_mass, b)], pw, rel, ns,
+                                                     sw, orbit_t[-1], rel[1] - rel[0])
+    assert np.all(ref_st == 1)
+    pert = sw.copy()
+    pert[:, 0] = np.nextafter(pert[:, 0], np.inf)
+    alt = O.mockstream_dop853(orc, [O.plummer(prog_mass, b)], pw, rel, ns, pert, orbit_t[-1],
+                              rel[1] - rel[0])[0]
+    pw2 = pw.copy()
+    pw2[:, 0] = np.nextafter(pw2[:, 0], np.inf)
+    alt2 = O.mockstream_dop853(orc, [O.plummer(prog_mass, b)], pw2, rel, ns, sw, orbit_t[-1],
+                               rel[1] - rel[0])[0]
+    sens = np.maximum(state_rel_err(alt.T, ref_out.T), state_rel_err(alt2.T, ref_out.T))
+    tol = np.maximum(1e-8, 100 * sens)
+    err = state_rel_err(stream.w(), ref_out.T)
+    print(f"dop853 stream: median {np.median(err):.3e}, max {err.max():.3e}; "
+          f"{np.sum(tol > 1e-8)} particle(s) gated")
+    assert np.median(err) < 1e-9
+    assert np.all(err <= tol)
+    np.testing.assert_allclose(prog.w()[:, 0], ref_pf, rtol=1e-8, atol=1e-12)

@@ -60,3 +60,29 @@ def test_dop853_backward_and_status_codes():
                        ref=True)
     assert st_a[0] == st_b[0] == -2
     assert np.array_equal(a, b)
+
+
+def test_mockstream_dop853_release_systems_bitwise():
+    """The oracle's DOP853 release systems (progenitor + particles, one shared
+    step, Fwrapper_direct_nbody forces, dop853_step's arguments) equal gala's
+    own compiled dop853() driven as mockstream_dop853 does, bit for bit,
+    including releases without particles and with one particle."""
+    if O.ref_lib() is None:
+        pytest.skip("reference build unavailable")
+    ext = O.milky_way_like()
+    prog = [O.plummer(1e5, 0.01)]
+    rng = np.random.default_rng(0)
+    nrel = 30
+    t1 = -300.0 + 10.0 * np.arange(nrel)
+    ns = np.full(nrel, 2, dtype=np.int32)
+    ns[0] = 0
+    ns[7] = 1
+    ph = 0.01 * np.arange(nrel)
+    pw = np.stack([15 * np.cos(ph), 15 * np.sin(ph), 0.3 + 0 * ph, -0.1 * np.sin(ph),
+                   0.18 * np.cos(ph), 0.03 + 0 * ph], axis=1)
+    sw = np.repeat(pw, ns, axis=0) + rng.normal(0, 0.05, (ns.sum(), 6)) * \
+        np.array([1, 1, 1, 0.01, 0.01, 0.01])
+    a = O.mockstream_dop853(ext, prog, pw, t1, ns, sw, 0.0, 10.0)
+    b = O.mockstream_dop853(ext, prog, pw, t1, ns, sw, 0.0, 10.0, ref=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2]) and a[2][0] == 0 and np.all(a[2][1:] == 1)
