@@ -84,7 +84,7 @@ def bench_dop853(args):
 
     def one():
         return dop853(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, order=ordd,
-                      block=args.block)
+                      n_tail=args.n_tail, block=args.block)
 
     for i in range(args.warmup):
         one()
@@ -126,7 +126,8 @@ def bench_dop853(args):
                                 "Miyamoto-Nagai 6.8e10/3.0/0.28 + NFW 5.4e11/15.6",
                    "rtol": 1e-8, "atol": 1e-10, "T": "logU(0.1, 10) Gyr per orbit",
                    "r0": "logU(1, 200) kpc", "step": "one integration of every orbit",
-                   "orbit_step": "accepted DOP853 step", "parallelism": "single GPU"},
+                   "orbit_step": "accepted DOP853 step", "parallelism": "single GPU",
+                   "queue": f"longest first, {args.n_tail} on dedicated 16-lane rows"},
         "accepted_steps": naccpt, "rejected_steps": nrejct,
         "max_attempts_one_orbit": int(stats[:, 0].max()),
         "fevals_per_s": nfcn / (ms_per_step / 1000.0),
@@ -230,6 +231,54 @@ def bench_leapfrog(args):
     return out
 
 
+def bench_mockstream(args):
+    """The mock stream of a Plummer progenitor (1e5 Msun, 10 pc) at (15, 0, 0)
+    kpc, (0, 180, 30) km/s in the composite: 4000 steps of dt = -1 Myr back,
+    2 particles per step (8002), each integrated forward to t = 0 with the
+    progenitor's gravity; gala's default DOP853 generator and the Leapfrog
+    one, end to end through MockStreamGenerator.run (host DF sampling
+    included), with the oracle's OpenMP release loop on the host beside the
+    Leapfrog kernel."""
+    import torch
+    from gala_amd.dynamics import FardalStreamDF, MockStreamGenerator
+    from gala_amd.potential import MilkyWayLikePotential, PlummerPotential
+    from gala_amd import workloads
+    prog_w0, prog_mass, b, dt, nsteps = workloads.mockstream_progenitor()
+    pot = MilkyWayLikePotential()
+    out = {}
+    for integ in ("leapfrog", "dop853"):
+        def run():
+            df = FardalStreamDF(gala_modified=True, random_state=np.random.RandomState(1))
+            gen = MockStreamGenerator(df, pot,
+                                      progenitor_potential=PlummerPotential(m=prog_mass, b=b))
+            return gen.run(prog_w0, prog_mass, dt=dt, n_steps=nsteps, Integrator=integ)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stream, _ = run()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        n = stream.xyz.shape[1]
+        # particle-steps of the reference's fixed-step loop: release i runs
+        # nsteps - i steps for each of its particles (and its progenitor copy)
+        psteps = int(2 * sum(range(nsteps + 1)))
+        out[integ] = {"particles": n, "wall_s": wall,
+                      "particle_steps_per_s": psteps / wall if integ == "leapfrog" else None}
+        log(f"mock stream ({integ}): {n} particles, {wall * 1e3:.1f} ms end to end")
+    if not args.no_cpu:
+        from oracle import oracle as O
+        rel_t = -np.arange(nsteps + 1)[::-1] * 1.0
+        full_t = rel_t
+        w0 = np.tile(prog_w0, (2 * (nsteps + 1), 1))
+        c0 = time.perf_counter()
+        O.mockstream_leapfrog(O.milky_way_like(), [O.plummer(prog_mass, b)], full_t, prog_w0,
+                              rel_t, np.full(nsteps + 1, 2, dtype=np.int32), w0, 0.0, fast=True)
+        cpu = time.perf_counter() - c0
+        out["leapfrog"]["cpu_release_loop_s"] = cpu
+        out["leapfrog"]["cpu_cores"] = int(O.num_threads(fast=True))
+    return out
+
+
 def bench_scf(args):
     """Exact SCF leapfrog (nmax 10, lmax 6, m 1e12, r_s 20) and the learned
     surrogate on the same 262,144 orbits x 10,000 steps of dt = 0.5 Myr."""
@@ -315,6 +364,8 @@ def main():
     ap.add_argument("--workload", default="composite_dop853")
     ap.add_argument("--n", type=int, default=0, help="override the orbit count (testing)")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--n-tail", type=int, default=1024,
+                    help="longest orbits integrated on dedicated 16-lane rows from the start")
     ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU baseline sample: every k-th orbit")
     ap.add_argument("--no-cpu", action="store_true")
@@ -323,6 +374,7 @@ def main():
     ap.add_argument("--no-leapfrog", action="store_true",
                     help="skip the composite leapfrog measurements")
     ap.add_argument("--lf-n", type=int, default=0, help="override the large leapfrog orbit count")
+    ap.add_argument("--no-stream", action="store_true", help="skip the mock-stream measurements")
     ap.add_argument("--scf-n", type=int, default=0, help="override the SCF orbit count")
     ap.add_argument("--scf-steps", type=int, default=0, help="override the SCF step count")
     ap.add_argument("--train-points", type=int, default=16 * 1024 * 1024)
@@ -341,6 +393,8 @@ def main():
     out, (w0, T) = bench_dop853(args)
     if not args.no_leapfrog:
         out["leapfrog_composite"] = bench_leapfrog(args)
+    if not args.no_stream:
+        out["mockstream"] = bench_mockstream(args)
     if not args.no_scf:
         out["scf_exact"], out["scf_surrogate"] = bench_scf(args)
     if not args.no_cpu:

@@ -56,15 +56,18 @@ enum { LAYOUT_GENERIC = 0, LAYOUT_MW = 1, LAYOUT_SCF106 = 2 };
 // twelve terms reach 1e-20; the series is summed by Estrin's scheme (depth
 // ~6 FMAs) and e ln2 is added with ln2 split in two (the high part has 21
 // trailing zero bits, so e * ln2_hi is exact).  Error ~1-2 ulp.
-__device__ __forceinline__ double log_fast(double x) {
-    int e;
+// argument reduction: x = 2^e (1 + f) with 1 + f in [sqrt(1/2), sqrt(2))
+__device__ __forceinline__ void log_reduce(double x, double &f, int &e) {
     double m = frexp(x, &e);  // m in [0.5, 1)
     if (m < 0.7071067811865476) {
         m = m + m;
         e -= 1;
     }
-    const double f = m - 1.0;
-    const double s = f / (2.0 + f);
+    f = m - 1.0;
+}
+
+// ln(2^e (1 + f)) from s = f / (2 + f)
+__device__ __forceinline__ double log_series(double s, int e) {
     const double z = s * s;
     const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
     const double p01 = fma(z, 0.3333333333333333, 1.0);
@@ -80,6 +83,13 @@ __device__ __forceinline__ double log_fast(double x) {
     const double ed = (double)e;
     const double lm = (s + s) * P;
     return fma(ed, 0.6931471803691238, fma(ed, 1.9082149292705877e-10, lm));
+}
+
+__device__ __forceinline__ double log_fast(double x) {
+    double f;
+    int e;
+    log_reduce(x, f, e);
+    return log_series(f / (2.0 + f), e);
 }
 
 // 1/sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64) refined by
@@ -287,6 +297,10 @@ __device__ __forceinline__ void comp_hessian_analytic(const DevComp &c, double x
 // coef layout: for l = 0..lmax, m = 0..l, n = 0..nmax: (S_nlm, T_nlm).
 // Returns Phi / (G M / r_s) scaled; accumulates G M/r_s^2 * grad into g.
 
+// 1/n for the Gegenbauer recurrences (n <= GAMD_SCF_MAX_NMAX + 1): a
+// wave-uniform table read instead of two FP64 divisions per (l, n) term
+static __constant__ double scf_inv_n[GAMD_SCF_MAX_NMAX + 2] = {0.0, 1.0 / 1.0, 1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0, 1.0 / 7.0, 1.0 / 8.0, 1.0 / 9.0, 1.0 / 10.0, 1.0 / 11.0, 1.0 / 12.0, 1.0 / 13.0, 1.0 / 14.0, 1.0 / 15.0, 1.0 / 16.0, 1.0 / 17.0, 1.0 / 18.0, 1.0 / 19.0, 1.0 / 20.0, 1.0 / 21.0};
+
 template <int NMAX, int LMAX>
 __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double G, double M,
                                            double rs, int nmax_rt, int lmax_rt, double x,
@@ -307,9 +321,11 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
         cphi = x / Rc;
         sphi = y / Rc;
     }
-    const double xi = (s - 1.0) / (s + 1.0);
     const double ops = 1.0 + s;
-    const double inv_ops2 = 1.0 / (ops * ops);
+    const double inv_ops = 1.0 / ops;
+    const double inv_s = 1.0 / s;
+    const double xi = (s - 1.0) * inv_ops;
+    const double inv_ops2 = inv_ops * inv_ops;
 
     double Phi = 0.0, dr = 0.0, dth = 0.0, dph = 0.0;
 
@@ -344,7 +360,7 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
         // radial functions for this l: phi_nl = pre * C_n^{a}(xi), a = 2l + 3/2
         // dphi_nl/ds = pre * [ (l/s - (2l+1)/(1+s)) C_n^{a} + 2a C_{n-1}^{a+1} 2/(1+s)^2 ]
         const double a = 2.0 * l + 1.5;
-        const double dlog = (l == 0 ? 0.0 : l / s) - (2.0 * l + 1.0) / ops;
+        const double dlog = (l == 0 ? 0.0 : l * inv_s) - (2.0 * l + 1.0) * inv_ops;
         const double chain = 4.0 * a * inv_ops2;
         // Y_lm for all m <= l at this l
         double ylm[LMAX >= 0 ? LMAX + 1 : GAMD_SCF_MAX_LMAX + 1];
@@ -373,11 +389,12 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
                 Cn = 2.0 * a * xi;
                 En = 1.0;
             } else {
-                Cn = (2.0 * xi * (n + a - 1.0) * c0 - (n + 2.0 * a - 2.0) * c1) / n;
+                Cn = (2.0 * xi * (n + a - 1.0) * c0 - (n + 2.0 * a - 2.0) * c1) * scf_inv_n[n];
                 const double b = a + 1.0;
                 const int k = n - 1;  // index of E
                 En = (k == 1) ? 2.0 * b * xi
-                              : (2.0 * xi * (k + b - 1.0) * e0 - (k + 2.0 * b - 2.0) * e1) / k;
+                              : (2.0 * xi * (k + b - 1.0) * e0 - (k + 2.0 * b - 2.0) * e1) *
+                                    scf_inv_n[k];
             }
             c1 = c0;
             c0 = Cn;
@@ -677,27 +694,39 @@ __device__ __forceinline__ void mw_grad(const DevPot &P, double x, double y, dou
     // (v_h2 / u^3 / r_s^2 = G m / r^3)
     const double u = r * nf.d[3];  // d[3] = 1 / r_s
     const double opu = 1.0 + u;
-    const double lg = log_fast(opu);
-    // The remaining denominators share one reciprocal (batched inversion:
-    // invert the product, peel the factors off with multiplies).
+    double lf;
+    int le;
+    log_reduce(opu, lf, le);
+    const double tf = 2.0 + lf;
+    // The remaining denominators -- the two Hernquist ones, 1 + u and the
+    // log's 2 + f -- share one reciprocal (batched inversion: invert the
+    // product, peel the factors off with multiplies).
     const double rc1 = r + h1.d[1];
     const double rc2 = r + h2.d[1];
     const double d1 = rc1 * rc1 * r, d2 = rc2 * rc2 * r;
     const double p12 = d1 * d2;
-    const double p = p12 * opu;
-    double f1, f2, f4;
-    if (p > 1e-280 && p < 1e280) {
+    const double p3 = p12 * opu;
+    const double p = p3 * tf;
+    const bool batched = p > 1e-280 && p < 1e280;
+    double f1, f2, f4, i4 = 0.0, slog;
+    if (batched) {
         const double inv = rcp_nr(p);
-        const double i4 = inv * p12, inv2 = inv * opu;
-        const double i1 = inv2 * d2, i2 = inv2 * d1;
-        f1 = h1.d[0] * i1;
-        f2 = h2.d[0] * i2;
-        f4 = nf.d[0] * nf.d[1] * (lg - u * i4) * ir3;
+        slog = lf * (inv * p3);
+        const double inv3 = inv * tf;
+        const double inv2 = inv3 * opu;
+        i4 = inv3 * p12;
+        f1 = h1.d[0] * (inv2 * d2);
+        f2 = h2.d[0] * (inv2 * d1);
     } else {  // extreme radii (or r = 0): the per-component forms
+        slog = lf / tf;
         f1 = h1.d[0] / d1;
         f2 = h2.d[0] / d2;
-        f4 = nf.d[0] / (u * u * u) / nf.d[2] * (lg - u / opu);
     }
+    const double lg = log_series(slog, le);
+    if (batched)
+        f4 = nf.d[0] * nf.d[1] * (lg - u * i4) * ir3;
+    else
+        f4 = nf.d[0] / (u * u * u) / nf.d[2] * (lg - u / opu);
     gx = 0.0;
     gy = 0.0;
     gz = 0.0;

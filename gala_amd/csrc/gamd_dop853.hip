@@ -49,6 +49,7 @@ struct DopArgs {
     unsigned long long *queue;
     int wide;              // finish the last orbits of a wave in 16-lane rows
     double uround;         // the "step size too small" threshold (dop853.cpp:349-354)
+    int64_t n_tail;        // queue slots [0, n_tail) go to dedicated wide waves
 };
 
 #define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
@@ -125,9 +126,10 @@ constexpr int WIDE_ORBITS = 4; // orbits per wave in the wide tail
 
 template <int SRC>
 __device__ __forceinline__ double row_bcast(double v) {
-    // DPP row_newbcast:SRC: every lane of a 16-lane row reads lane SRC of the row
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + SRC, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + SRC, 0xF, 0xF, false);
+    // DPP row_newbcast:SRC: every lane of a 16-lane row reads lane SRC of the
+    // row (every lane has a source in its row, so no "old" value is needed)
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + SRC, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + SRC, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -142,8 +144,8 @@ __device__ __forceinline__ double row_shl3(double v) {
 // DPP quad_perm [B,B,B,B]: every lane of a 4-lane quad reads lane B of the quad
 template <int B>
 __device__ __forceinline__ double quad_bcast(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x55 * B, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x55 * B, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x55 * B, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x55 * B, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -526,6 +528,30 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
     }
 }
 
+// start the orbit at queue slot `slot`: state, first derivative and, for
+// h0 = 0, Hairer's initial step (dopcor's set-up, dop853.cpp:300-320)
+template <bool SCF, int LAY>
+__device__ __forceinline__ void load_orbit(const DevPot &P, const DopArgs &a, int64_t slot,
+                                           DopState &s, double (&y)[6], double (&k1)[6]) {
+    int64_t orb = a.order ? (int64_t)a.order[slot] : slot;
+    if (orb < 0 || orb >= a.N) orb = slot;  // malformed order: fall back
+    s.orb = orb;
+    FOR6 y[c] = a.w0[c * a.N + orb];
+    s.x = a.t0[orb];
+    s.xend = a.t1[orb];
+    s.h = a.h0 ? a.h0[orb] : 0.0;
+    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(s.xend - s.x);  // dop853.cpp:787-788, 312
+    s.posneg = sign_d(1.0, s.xend - s.x);
+    s.facold = 1.0e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nfcn = 2;
+    s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = 0;
+    orbit_rhs<SCF, LAY>(P, y, k1);
+    if (s.h == 0.0) s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+}
+
 template <bool SCF, bool DENSE, int LAY>
 __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a) {
     const DevPot &P = *Pp;
@@ -546,6 +572,22 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         k1[c] = 0.0;
     }
 
+    if (!DENSE && a.n_tail > 0) {
+        // the first waves of the grid take the queue order's first n_tail
+        // (longest) orbits, four per wave, and integrate them in 16-lane rows
+        // from the start; the other waves share the rest of the queue
+        const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        if (wave * WIDE_ORBITS < a.n_tail) {
+            const int64_t slot = wave * WIDE_ORBITS + lane;
+            if (lane < WIDE_ORBITS && slot < a.n_tail) {
+                load_orbit<SCF, LAY>(P, a, slot, s, y, k1);
+                active = true;
+            }
+            wide_tail<SCF, LAY>(P, a, active, s, y, k1);
+            return;
+        }
+    }
+
     while (true) {
         // ---- refill idle lanes from the queue (one atomic per wave) ----
         const bool need = !active && !exhausted;
@@ -558,25 +600,9 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
             if (need) {
                 const unsigned long long below = want & ((1ull << lane) - 1ull);
                 const unsigned long long slot = base + (unsigned long long)__popcll(below);
-                if (slot < (unsigned long long)a.N) {
-                    int64_t orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
-                    if (orb < 0 || orb >= a.N) orb = (int64_t)slot;  // malformed order: fall back
-                    s.orb = orb;
-                    FOR6 y[c] = a.w0[c * a.N + orb];
-                    s.x = a.t0[orb];
-                    s.xend = a.t1[orb];
-                    s.h = a.h0 ? a.h0[orb] : 0.0;
-                    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(s.xend - s.x);  // dop853.cpp:787-788, 312
-                    s.posneg = sign_d(1.0, s.xend - s.x);
-                    s.facold = 1.0e-4;
-                    s.hlamb = 0.0;
-                    s.nstep = s.naccpt = s.nrejct = 0;
-                    s.nfcn = 2;
-                    s.oidx = 0;
-                    s.reject = s.iasti = s.nonsti = 0;
-                    orbit_rhs<SCF, LAY>(P, y, k1);
-                    if (s.h == 0.0)
-                        s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+                const unsigned long long qslot = slot + (unsigned long long)a.n_tail;
+                if (qslot < (unsigned long long)a.N) {
+                    load_orbit<SCF, LAY>(P, a, (int64_t)qslot, s, y, k1);
                     active = true;
                 } else {
                     exhausted = true;
@@ -691,7 +717,14 @@ int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
     const int64_t needed = (a.N + block - 1) / block;
     if (blocks > needed) blocks = needed;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, s, P, a);
+    DopArgs b = a;
+    // dedicated wide waves: at most half the grid's waves, never all of it
+    const int64_t waves = blocks * (block / 64);
+    int64_t max_tail = WIDE_ORBITS * (waves / 2);
+    if (DENSE || !a.wide) max_tail = 0;
+    if (b.n_tail > max_tail) b.n_tail = max_tail;
+    if (b.n_tail > a.N) b.n_tail = a.N;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, s, P, b);
     return launch_status();
 }
 
@@ -701,9 +734,11 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
                            const double *t1, const double *h0, double rtol, double atol,
                            int64_t nmax, int64_t nstiff, double hmax, const double *t_out,
                            size_t n_out, double *w_out, double *w_final, int32_t *status,
-                           int64_t *stats, const int32_t *order, int block, void *stream) {
+                           int64_t *stats, const int32_t *order, int64_t n_tail, int block,
+                           void *stream) {
     if (!pot || !w0 || !t0 || !t1 || !w_final || (w_out && (!t_out || n_out == 0)))
         return GAMD_ERR_INVALID;
+    if (n_tail < 0) return GAMD_ERR_INVALID;
     if (block == 0) block = 256;
     if (block < 64 || block > 1024 || block % 64) return GAMD_ERR_INVALID;
     if (nmax == 0) nmax = 1000000;                   // dop853.cpp:673-674
@@ -733,6 +768,7 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     a.status = status;
     a.stats = stats;
     a.order = order;
+    a.n_tail = n_tail;
     a.queue = queue;
     // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end (same results)
     const char *wide_env = getenv("GAMD_DOP853_WIDE");

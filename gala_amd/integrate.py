@@ -139,9 +139,11 @@ def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
 
 
 def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0, hmax=0.0,
-           t_out=None, order=None, block=0):
+           t_out=None, order=None, n_tail=0, block=0):
     """Per-orbit adaptive DOP853 on device.  w0 (6, N); t0, t1, h0 per orbit
-    (scalars broadcast).  Returns dict(w_final, status, stats[, w_out])."""
+    (scalars broadcast); order: queue order; n_tail: the first n_tail orbits
+    of that order run on dedicated 16-lane rows from the start.
+    Returns dict(w_final, status, stats[, w_out])."""
     t = dev.require_gpu()
     lib = _lib.load()
     w0 = dev.to_dev(w0)
@@ -168,7 +170,8 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
                                int(nstiff), float(hmax), dev.ptr(tout),
                                0 if tout is None else tout.shape[0], dev.ptr(wout),
                                wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
-                               dev.ptr(ordd), int(block), dev.stream_ptr()), "gamd_dop853")
+                               dev.ptr(ordd), int(n_tail), int(block), dev.stream_ptr()),
+               "gamd_dop853")
     out = {"w_final": wfin, "status": status, "stats": stats}
     if wout is not None:
         out["w_out"] = wout

@@ -117,7 +117,7 @@ def mockstream_progenitor():
     return w0, 1e5, 0.01, -1.0, 4000
 
 
-def cost_order(w0, T, gradient_fn, spread=64, per_wave=4, n_spread=2048):
+def cost_order(w0, T, gradient_fn, spread=1, per_wave=4, n_spread=2048):
     """Longest-first queue order for the DOP853 work queue: the estimated
     number of dynamical times T_i * sqrt(|g| / r) per orbit, descending.
 
@@ -127,7 +127,9 @@ def cost_order(w0, T, gradient_fn, spread=64, per_wave=4, n_spread=2048):
     slots each, so each of n_spread / per_wave waves starts with per_wave of
     them among cheaper orbits, holds only those once the queue is drained,
     and finishes them in the kernel's wide tail (up to four orbits on 16-lane
-    rows).  spread <= 1 or n_spread = 0 keeps the plain longest-first order."""
+    rows).  spread <= 1 (the default) or n_spread = 0 keeps the plain
+    longest-first order, the one to pair with gamd_dop853's n_tail, which
+    gives the head of the order dedicated 16-lane rows from the start."""
     q = w0[:3]
     r = np.sqrt(np.sum(q * q, axis=0))
     g = gradient_fn(q)

@@ -131,12 +131,16 @@ int gamd_ruth4(const gamd_potential *pot, const double *w0, size_t N, double t0,
  * (N,4) = nstep, naccpt, nrejct, nfcn (nullable) are per orbit.
  * order (device, nullable): the order in which the persistent kernel's work
  * queue hands orbits to lanes (a permutation of 0..N-1; longest first keeps
- * the tail short).  block: threads per workgroup (0 = autotuned). */
+ * the tail short).  n_tail >= 0: the first n_tail orbits of that order are
+ * integrated from the start by dedicated waves, four orbits per wave, each
+ * spread over 16 lanes (capped at half the grid; ignored with dense output);
+ * results are identical for every n_tail.  block: threads per workgroup
+ * (0 = autotuned). */
 int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
                 const double *t1, const double *h0, double rtol, double atol, int64_t nmax,
                 int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
                 double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
-                int block, void *stream);
+                int64_t n_tail, int block, void *stream);
 
 /* ---- mock streams ------------------------------------------------------- */
 /* Replaces mockstream_leapfrog's release-and-integrate loop

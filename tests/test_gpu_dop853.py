@@ -133,3 +133,24 @@ def test_wide_tail_matches_one_lane_per_orbit(layout, monkeypatch):
         for a, b in zip(out["0", nstiff], out["1", nstiff]):
             assert np.array_equal(a, b)
     assert np.all(out["1", 0][1] == 1)
+
+
+def test_dedicated_tail_waves_match():
+    """n_tail > 0 runs the queue order's first orbits on dedicated 16-lane
+    rows from the start; the results are those of the one-lane kernel, bit
+    for bit, for the longest-first order and for a shuffled one."""
+    from gala_amd.integrate import dop853
+    from gala_amd import _device as dev
+    orc = mw_oracle()
+    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=6000, seed=23)
+    T = T * 0.05
+    pot = mw_product()
+    base = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
+    ref = [dev.to_host(base[k]) for k in ("w_final", "status", "stats")]
+    lpt = workloads.cost_order(w0, T, pot.gradient, spread=1)
+    shuffled = np.random.default_rng(1).permutation(w0.shape[1]).astype(np.int32)
+    for order in (lpt, shuffled):
+        for n_tail in (4, 37, 512, 100000):
+            r = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, order=order, n_tail=n_tail)
+            for k, a in zip(("w_final", "status", "stats"), ref):
+                assert np.array_equal(dev.to_host(r[k]), a), (k, n_tail)

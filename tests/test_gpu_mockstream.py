@@ -160,7 +160,9 @@ def test_full_size_stream_against_oracle():
         print(f"  particle {i}: err {err[i]:.3e} tol {tol[i]:.3e} release {ics.release_time[i]}")
     assert np.median(err) < 1e-10
     assert len(bad) == 0
-    np.testing.assert_allclose(prog.w()[:, 0], pf, rtol=1e-9, atol=1e-12)
+    # the progenitor returns to (15, 0, 0) kpc: compare it relative to |x|, |v|
+    # (its near-zero y and z components differ at the 1e-12 kpc level)
+    assert state_rel_err(prog.w()[:, 0:1], pf[:, None])[0] < 1e-10
 
 
 def _dop853_release_inputs(nrel=120, seed=5):
@@ -265,4 +267,4 @@ def test_dop853_generator_against_oracle_pipeline():
           f"{np.sum(tol > 1e-8)} particle(s) gated")
     assert np.median(err) < 1e-9
     assert np.all(err <= tol)
-    np.testing.assert_allclose(prog.w()[:, 0], ref_pf, rtol=1e-8, atol=1e-12)
+    assert state_rel_err(prog.w()[:, 0:1], ref_pf[:, None])[0] < 1e-8

@@ -42,7 +42,10 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     assert lib.gamd_potential_create(None, -1, ctypes.byref(out)) == -1
     assert lib.gamd_leapfrog(None, None, 1, 0.0, 1.0, 1, None, 1, None, None, None) == -1
     assert lib.gamd_dop853(None, None, 1, None, None, None, 1e-8, 1e-10, 0, 0, 0.0, None, 0, None,
-                           None, None, None, None, 0, None) == -1
+                           None, None, None, None, 0, 0, None) == -1
+    assert lib.gamd_ruth4(None, None, 1, 0.0, 1.0, 1, 1, None, None, None) == -1
+    assert lib.gamd_mockstream_dop853(None, None, None, 1, None, None, None, None, 0.0, 1.0,
+                                      1e-10, 1e-10, 0, None, None, None, None, None) == -1
     assert lib.gamd_mockstream_leapfrog(None, None, None, 0, 0.0, 1.0, None, None, 0, 0.0, None,
                                         None) == -1
     assert lib.gamd_potential_destroy(None) == 0
