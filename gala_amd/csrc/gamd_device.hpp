@@ -308,6 +308,11 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
                                            double &gx, double &gy, double &gz) {
     const int nmax = NMAX >= 0 ? NMAX : nmax_rt;
     const int lmax = LMAX >= 0 ? LMAX : lmax_rt;
+    // the coefficients are read-only for the whole launch and every lane reads
+    // the same one: through the constant address space they become scalar
+    // (SMEM) loads instead of a vector load and wait per term
+    const __attribute__((address_space(4))) double *cfc =
+        (const __attribute__((address_space(4))) double *)cf;
     const double FOURPI_SQRT = 3.544907701811031;
     const double R2 = x * x + y * y;
     const double r2 = R2 + z * z;
@@ -405,8 +410,8 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
 #pragma unroll
             for (int m = 0; m <= (LMAX >= 0 ? LMAX : GAMD_SCF_MAX_LMAX); m++) {
                 if (m <= l) {
-                    const double S = cf[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n)];
-                    const double T = cf[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n) + 1];
+                    const double S = cfc[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n)];
+                    const double T = cfc[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n) + 1];
                     As[m] = fma(phinl, S, As[m]);
                     Bs[m] = fma(phinl, T, Bs[m]);
                     dAs[m] = fma(dphinl, S, dAs[m]);
