@@ -53,9 +53,16 @@ constexpr int NPARAM = OFF_B5 + 4;  // 199,684
 // BF16 copies: swizzled W2..W4, then row-major W2..W4
 constexpr int NBF = 6 * H * H;
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// logistic function from the hardware exp2 and reciprocal (v_exp_f32,
+// v_rcp_f32, ~1 ulp each): the activations are rounded to BF16 (8 bits)
+// right after, so an IEEE divide's extra accuracy is never seen.  Large
+// negative z: exp2 overflows to +inf and the reciprocal gives 0, the limit.
+__device__ __forceinline__ float sigmoid_fast(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid_fast(z);
     return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ __bf16 tobf(float f) { return (__bf16)f; }
