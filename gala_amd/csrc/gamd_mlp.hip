@@ -101,7 +101,7 @@ __device__ __forceinline__ void out_scales(const gamd_mlp_norm &nm, double r, do
 // raw outputs are in LDS out[TILE][4].  With TRAIN, pre-activations and
 // activations of the four hidden layers go to global (BF16, [4][M][256]).
 template <bool TRAIN>
-__device__ void tile_forward(const float *__restrict__ theta, const __bf16 *__restrict__ wsw,
+__device__ __forceinline__ void tile_forward(const float *__restrict__ theta, const __bf16 *__restrict__ wsw,
                              const float (*feat)[4], __bf16 *bufA, __bf16 *bufB, float (*out)[4],
                              int64_t row0, int64_t M, __bf16 *__restrict__ Zs,
                              __bf16 *__restrict__ Hs) {
