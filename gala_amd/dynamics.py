@@ -292,8 +292,10 @@ class DirectNBody:
     """Massive bodies in an external potential (dynamics/nbody/core.py).  Only
     the case the stream generator builds is supported: one massive body (the
     progenitor), whose own potential does not act on itself, so its orbit is a
-    test-particle orbit in the external potential (leapfrog.pyx:126-257 with
-    nbody_i == 0 skipped, cpotential.cpp:430)."""
+    test-particle orbit in the external potential: Leapfrog
+    (leapfrog.pyx:126-257 with nbody_i == 0 skipped, cpotential.cpp:430) or,
+    gala's default, DOP853 (direct_nbody_dop853, nbody.pyx:30-115: dense
+    output on t, h0 = t[1] - t[0], stiffness test off)."""
 
     def __init__(self, w0, particle_potentials, external_potential=None, frame=None, units=None,
                  save_all=True):
@@ -312,10 +314,26 @@ class DirectNBody:
         self.save_all = save_all
 
     def integrate_orbit(self, Integrator=None, Integrator_kwargs=None, **time_spec):
-        Integrator = get_integrator(Integrator or LeapfrogIntegrator)
-        if Integrator is not LeapfrogIntegrator:
-            raise NotImplementedError("only Leapfrog N-body integration is supported")
+        Integrator = get_integrator(Integrator or DOPRI853Integrator)
         t = parse_time_specification(self.units, **time_spec)
+        if Integrator is DOPRI853Integrator:
+            from .integrate import dop853
+            kw = dict(Integrator_kwargs or {})
+            res = dop853(self.external_potential, self.w0, t[0], t[-1], t[1] - t[0],
+                         rtol=float(kw.get("rtol", 1e-10)), atol=float(kw.get("atol", 1e-10)),
+                         nmax=int(kw.get("nmax", 0)), nstiff=-1,
+                         hmax=float(kw.get("dt_max", 0.0)),
+                         t_out=t if self.save_all else None)
+            st = dev.to_host(res["status"])
+            if int(kw.get("err_if_fail", 1)) == 1 and np.any(st < 0):
+                raise RuntimeError(f"Integration failed with code {int(st[st < 0][0])}")
+            if self.save_all:
+                w = dev.to_host(res["w_out"])  # (6, ntimes, 1)
+                return Orbit(pos=w[:3], vel=w[3:], t=t, hamiltonian=self.H)
+            w = dev.to_host(res["w_final"])
+            return PhaseSpacePosition(w[:3], w[3:])
+        if Integrator is not LeapfrogIntegrator:
+            raise NotImplementedError("N-body integration supports Leapfrog and DOP853")
         wout, wfin = leapfrog(self.external_potential, self.w0, t[1] - t[0], len(t) - 1,
                               save=self.save_all)
         if self.save_all:
