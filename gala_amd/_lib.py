@@ -13,6 +13,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgala_amd.so")
 
 MAX_COMPONENTS = 8
+MAX_BODIES = 64   # GAMD_MAX_BODIES: bodies of one DOP853 N-body system
+MAX_MASSIVE = 8   # GAMD_MAX_MASSIVE: bodies carrying a potential
 SCF_MAX_NMAX = 20
 SCF_MAX_LMAX = 10
 
@@ -56,10 +58,14 @@ SIGNATURES = {
     "gamd_ruth4": (_int, [_vp, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp, _vp]),
     "gamd_dop853": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _d, _d, _i64, _i64, _d, _vp, _sz,
                            _vp, _vp, _vp, _vp, _vp, _i64, _int, _vp]),
-    "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _d, _vp,
-                                        _vp]),
-    "gamd_mockstream_dop853": (_int, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _d, _d, _d, _d,
-                                      _i64, _vp, _vp, _vp, _vp, _vp]),
+    "gamd_nbody_leapfrog": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp,
+                                   _vp]),
+    "gamd_nbody_dop853": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _d, _d, _d, _i64, _i64, _d,
+                                 _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _vp, _vp, _sz, _d,
+                                        _vp, _vp]),
+    "gamd_mockstream_dop853": (_int, [_vp, _vp, _int, _vp, _sz, _vp, _vp, _vp, _vp, _d, _d, _d,
+                                      _d, _i64, _vp, _vp, _vp, _vp, _vp]),
     "gamd_mlp_param_count": (_i64, []),
     "gamd_mlp_bf16_count": (_i64, []),
     "gamd_mlp_workspace_bytes": (_i64, [_i64]),
@@ -121,3 +127,12 @@ def make_components(components):
             cc.S, cc.T = S.ctypes.data, T.ctypes.data
             keep += [S, T]
     return arr, keep
+
+
+def body_array(handles):
+    """gamd_potential* array (host) for the body arguments of the N-body entry
+    points: one handle per body, None for a massless body."""
+    arr = (C.c_void_p * max(len(handles), 1))()
+    for i, h in enumerate(handles):
+        arr[i] = h
+    return arr

@@ -18,9 +18,16 @@
 //   * the queue hands out orbits in a caller-chosen order (longest first
 //     keeps the tail short); the loop ends when the queue is exhausted for
 //     every lane of the wave, which every wave reaches.
+//
+// The same step-attempt body also integrates N-body systems with
+// Fwrapper_direct_nbody forces (dop853.cpp:990-1006): mockstream_dop853's
+// release systems (gamd_mockstream_dop853) and direct_nbody_dop853
+// (gamd_nbody_dop853), one body per lane of a quad or a wave.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include <type_traits>
 
 #include "dop853_tableau.h"
 #include "gamd_host.hpp"
@@ -149,6 +156,14 @@ __device__ __forceinline__ double quad_bcast(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// per-orbit scalars of the integration (wave-uniform within a wide row)
+struct DopState {
+    int64_t orb;
+    double x, xend, h, hmax, posneg, facold, hlamb;
+    int64_t nstep, naccpt, nrejct, nfcn, oidx;
+    int reject, iasti, nonsti;
+};
+
 // Lane layouts of one DOP853 system (the policy of dop_attempt):
 //   rhs   the lane's share of f(y);
 //   sums  sa += a_i^2, sb += b_i^2 over the system's n components in the
@@ -170,6 +185,12 @@ struct OneLane {
         }
     }
     __device__ __forceinline__ double n() const { return 6.0; }
+    // dense sample s.oidx of orbit s.orb into w_out (6, n_out, N)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[6]) const {
+        const int64_t so = a.n_out * a.N;
+        FOR6 a.w_out[c * so + s.oidx * a.N + s.orb] = v[c];
+    }
 };
 
 // one orbit per 16-lane row, lane c holding component comp
@@ -206,28 +227,45 @@ struct WideRow {
     __device__ __forceinline__ double n() const { return 6.0; }
 };
 
-// one mock-stream release system per 4-lane quad (Fwrapper_direct_nbody,
-// dop853.cpp:990-1006): lane b holds body b's six components, body 0 the
-// progenitor, bodies 1..nb-1 massless particles that also feel the
-// progenitor's potential PP centred on body 0 (c_nbody_acceleration,
-// cpotential.cpp:389-415).  Lanes b >= nb mirror body 0 and are left out of
-// every sum.  The state vector is body-major, as the reference's (nb, 6) rows.
+// N-body systems (Fwrapper_direct_nbody, dop853.cpp:990-1006): lane b of a
+// group holds body b's six components; the state vector is body-major, as the
+// reference's (nb, 6) rows.  Every body feels the external potential, then
+// the potential of every other body j < bt.n that has one, centred on body j
+// (c_nbody_acceleration, cpotential.cpp:389-415), accumulated in body order;
+// f = -(g_ext + f_0 + f_1 + ...) equals the reference's
+// (-g_ext) + (-f_0) + (-f_1) + ... exactly.  Lanes b >= nb mirror body 0 and
+// are left out of every sum.
+//   QuadBodies: one system of at most 4 bodies per 4-lane quad (several
+//               systems per wave), DPP quad_perm broadcasts;
+//   WaveBodies: one system of at most 64 bodies per wave, readlane
+//               broadcasts (wave-uniform source lane).
 template <bool SCF, int LAY>
 struct QuadBodies {
     static constexpr int NC = 6;
     int body, nb;
-    const DevPot *PP;  // nullptr: the progenitor exerts no force
+    const BodyTable &bt;  // bt.n <= nb <= 4
+    template <int J>
+    __device__ __forceinline__ void add_body(const double (&y)[6], double &gx, double &gy,
+                                             double &gz) const {
+        if (J < bt.n) {  // wave-uniform
+            const double px = quad_bcast<J>(y[0]), py = quad_bcast<J>(y[1]),
+                         pz = quad_bcast<J>(y[2]);
+            if (J != body && body < nb && bt.p[J]) {
+                double fx, fy, fz;
+                body_grad<SCF>(*bt.p[J], y[0], y[1], y[2], px, py, pz, fx, fy, fz);
+                gx += fx;
+                gy += fy;
+                gz += fz;
+            }
+        }
+    }
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
-        const double px = quad_bcast<0>(y[0]), py = quad_bcast<0>(y[1]), pz = quad_bcast<0>(y[2]);
         double gx, gy, gz;
         pot_grad_l<SCF, LAY>(P, y[0], y[1], y[2], gx, gy, gz);
-        if (PP && body >= 1 && body < nb) {
-            double fx, fy, fz;
-            body_grad<SCF>(*PP, y[0], y[1], y[2], px, py, pz, fx, fy, fz);
-            gx += fx;
-            gy += fy;
-            gz += fz;
-        }
+        add_body<0>(y, gx, gy, gz);
+        add_body<1>(y, gx, gy, gz);
+        add_body<2>(y, gx, gy, gz);
+        add_body<3>(y, gx, gy, gz);
         f[0] = y[3];
         f[1] = y[4];
         f[2] = y[5];
@@ -260,12 +298,56 @@ struct QuadBodies {
     __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
 };
 
-// per-orbit scalars of the integration (wave-uniform within a wide row)
-struct DopState {
-    int64_t orb;
-    double x, xend, h, hmax, posneg, facold, hlamb;
-    int64_t nstep, naccpt, nrejct, nfcn, oidx;
-    int reject, iasti, nonsti;
+// v of lane src (wave-uniform) in every lane
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
+template <bool SCF, int LAY>
+struct WaveBodies {
+    static constexpr int NC = 6;
+    int body, nb;
+    const BodyTable &bt;  // bt.n <= nb <= 64
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(P, y[0], y[1], y[2], gx, gy, gz);
+        for (int j = 0; j < bt.n; j++) {
+            const double px = lane_bcast(y[0], j), py = lane_bcast(y[1], j),
+                         pz = lane_bcast(y[2], j);
+            if (j != body && body < nb && bt.p[j]) {
+                double fx, fy, fz;
+                body_grad<SCF>(*bt.p[j], y[0], y[1], y[2], px, py, pz, fx, fy, fz);
+                gx += fx;
+                gy += fy;
+                gz += fz;
+            }
+        }
+        f[0] = y[3];
+        f[1] = y[4];
+        f[2] = y[5];
+        f[3] = -gx;
+        f[4] = -gy;
+        f[5] = -gz;
+    }
+    __device__ __forceinline__ void sums(const double (&a)[6], const double (&b)[6], double &sa,
+                                         double &sb) const {
+        for (int j = 0; j < nb; j++) {
+            FOR6 {
+                const double qa = lane_bcast(a[c], j), qb = lane_bcast(b[c], j);
+                sa += qa * qa;
+                sb += qb * qb;
+            }
+        }
+    }
+    __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
+    // dense sample s.oidx of body `body` into w_out (6, n_out, nb)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[6]) const {
+        const int64_t so = a.n_out * a.N;
+        if (body < nb) FOR6 a.w_out[c * so + s.oidx * a.N + body] = v[c];
+    }
 };
 
 #define FORN _Pragma("unroll") for (int c = 0; c < NC; c++)
@@ -399,17 +481,17 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                                          T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
                                          T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
                                          T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
-                orbit_rhs<SCF, LAY>(P, yy, k10);
+                sys.rhs(P, yy, k10);
                 FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
                                          T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
                                          T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
                                          T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
-                orbit_rhs<SCF, LAY>(P, yy, k2);
+                sys.rhs(P, yy, k2);
                 FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
                                          T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
                                          T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
                                          T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
-                orbit_rhs<SCF, LAY>(P, yy, k3);
+                sys.rhs(P, yy, k3);
                 s.nfcn += 3;
                 FOR6 {
                     r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
@@ -422,14 +504,14 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                                  T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
                 }
                 const double x0 = s.x, x1 = s.x + h;
-                const int64_t so = a.n_out * a.N;
                 while (s.oidx < a.n_out) {
                     const double to = a.t_out[s.oidx];
                     if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
                         const double sv = (to - s.x) / h, s1 = 1.0 - sv;
-                        FOR6 a.w_out[c * so + s.oidx * a.N + s.orb] =
-                            r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
-                            sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
+                        double v[6];
+                        FOR6 v[c] = r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
+                                    sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
+                        sys.store_dense(a, s, v);
                         s.oidx++;
                     } else {
                         break;
@@ -631,47 +713,58 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
 
 // ---------------------------------------------------------------------------
 // mockstream_dop853's release systems (mockstream.pyx:286-303): system i is
-// the progenitor at its release state plus the release's particles, one
-// (1 + k, 6) system integrated by dop853_step (dop853.pyx:29-79) from
-// rel_t[i] to tfinal; one system per 4-lane quad, all systems concurrently.
+// the bodies (progenitor first, then any perturbers) at their states at the
+// release plus the release's particles, one (K + k, 6) system integrated by
+// dop853_step (dop853.pyx:29-79) from rel_t[i] to tfinal; all systems run
+// concurrently.  Systems of at most 4 bodies run on 4-lane quads (16 per
+// wave); larger ones (more perturbers or particles per release) take a wave
+// each, in a second launch.
 struct StreamArgs {
-    const double *prog_w;     // (nrel, 6) progenitor state at each release
+    const double *body_w;     // (nrel, K, 6) bodies at each release
     const double *rel_t;      // (nrel,)
-    const int32_t *nstream;   // (nrel,) particles released (0..3)
+    const int32_t *nstream;   // (nrel,) particles released
     const int64_t *offset;    // (nrel,) first particle row of each release
     const double *stream_w0;  // (ntot, 6)
     int64_t nrel;
     double tfinal, dt0;
     double *out;              // (ntot, 6)
-    double *prog_final;       // (nrel, 6)
+    double *body_final;       // (nrel, K, 6)
     int32_t *status;          // (nrel,)
     int64_t *stats;           // (nrel, 4) or nullptr
 };
 
-constexpr int STREAM_MAX_PARTICLES = 3;  // a quad holds the progenitor and three particles
-
-template <bool SCF, int LAY>
-__global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict__ Pp,
-                                                       const DevPot *__restrict__ PPp, int has_prog,
-                                                       StreamArgs sa, DopArgs a) {
+// G = 4: QuadBodies; G = 64: WaveBodies.  quads: the quad launch takes the
+// systems of at most 4 bodies (so the wave launch skips them).
+template <bool SCF, int LAY, int G>
+__global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict__ Pp, BodyTable bt,
+                                                       int quads, StreamArgs sa, DopArgs a) {
     const DevPot &P = *Pp;
     const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t rel = gid >> 2;  // uniform over the quad
-    const int body = (int)(gid & 3);
+    const int64_t rel = gid / G;  // uniform over the group
+    const int body = (int)(gid % G);
     if (rel >= sa.nrel) return;
+    const int K = bt.n;
     const int k = sa.nstream[rel];
-    if (k <= 0 || k > STREAM_MAX_PARTICLES) {
-        if (body == 0) sa.status[rel] = k <= 0 ? 0 : GAMD_ERR_UNSUPPORTED;
+    if (k <= 0) {
+        if (body == 0 && (G == 4 || !quads)) sa.status[rel] = 0;
         return;
     }
-    const int nb = 1 + k;
-    const QuadBodies<SCF, LAY> sys{body, nb, has_prog ? PPp : nullptr};
+    const int nb = K + k;
+    if (G == 4 && nb > 4) return;              // a wave's system
+    if (G == 64 && quads && nb <= 4) return;   // a quad's system
+    if (nb > G) {
+        if (body == 0) sa.status[rel] = GAMD_ERR_UNSUPPORTED;
+        return;
+    }
+    using Sys = typename std::conditional<G == 4, QuadBodies<SCF, LAY>, WaveBodies<SCF, LAY>>::type;
+    const Sys sys{body, nb, bt};
     double y[6], k1[6];
-    if (body >= 1 && body < nb) {
-        const double *src = sa.stream_w0 + 6 * (sa.offset[rel] + body - 1);
+    if (body >= K && body < nb) {
+        const double *src = sa.stream_w0 + 6 * (sa.offset[rel] + body - K);
         FOR6 y[c] = src[c];
     } else {
-        FOR6 y[c] = sa.prog_w[6 * rel + c];
+        const int b = body < K ? body : 0;
+        FOR6 y[c] = sa.body_w[6 * (rel * K + b) + c];
     }
     DopState s;
     s.orb = rel;
@@ -688,14 +781,14 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
     s.reject = s.iasti = s.nonsti = 0;
     sys.rhs(P, y, k1);
     int code = 0;
-    // every quad ends: each attempt advances nstep, and nmax bounds it
+    // every group ends: each attempt advances nstep, and nmax bounds it
     while (code == 0) code = dop_attempt<SCF, false, LAY>(P, a, sys, s, y, k1);
-    if (body >= 1 && body < nb) {
-        double *dst = sa.out + 6 * (sa.offset[rel] + body - 1);
+    if (body >= K && body < nb) {
+        double *dst = sa.out + 6 * (sa.offset[rel] + body - K);
         FOR6 dst[c] = y[c];
     }
+    if (body < K) FOR6 sa.body_final[6 * (rel * K + body) + c] = y[c];
     if (body == 0) {
-        FOR6 sa.prog_final[6 * rel + c] = y[c];
         sa.status[rel] = code;
         if (sa.stats) {
             sa.stats[4 * rel + 0] = s.nstep;
@@ -704,6 +797,39 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
             sa.stats[4 * rel + 3] = s.nfcn;
         }
     }
+}
+
+// direct_nbody_dop853 (nbody.pyx:30-115): the a.N bodies of w0 (6, N) as one
+// system on one wave, dense output at t_out (dop853_helper, dop853.pyx:90-193).
+template <bool SCF, bool DENSE, int LAY>
+__global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ Pp, BodyTable bt,
+                                                     double t0, double t1, double h0, DopArgs a) {
+    const DevPot &P = *Pp;
+    const int body = threadIdx.x;
+    const int nb = (int)a.N;
+    const WaveBodies<SCF, LAY> sys{body, nb, bt};
+    const int src = body < nb ? body : 0;
+    double y[6], k1[6];
+    FOR6 y[c] = a.w0[c * a.N + src];
+    DopState s;
+    s.orb = 0;
+    s.x = t0;
+    s.xend = t1;
+    s.h = h0;
+    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(t1 - t0);  // dop853.cpp:787-788
+    s.posneg = sign_d(1.0, t1 - t0);
+    s.facold = 1.0e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nfcn = 2;
+    s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = 0;
+    sys.rhs(P, y, k1);
+    int code = 0;
+    // the wave ends: each attempt advances nstep, and nmax bounds it
+    while (code == 0) code = dop_attempt<SCF, DENSE, LAY>(P, a, sys, s, y, k1);
+    if (body < nb) FOR6 a.w_final[c * a.N + body] = y[c];
+    if (body == 0) finish_stats(a, s, code);
 }
 
 template <bool SCF, bool DENSE, int LAY>
@@ -790,14 +916,18 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     return rc;
 }
 
-extern "C" int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *prog,
-                                      const double *prog_w, size_t nrel, const double *rel_t,
+extern "C" int gamd_mockstream_dop853(const gamd_potential *ext,
+                                      const gamd_potential *const *bodies, int nbody,
+                                      const double *body_w, size_t nrel, const double *rel_t,
                                       const int32_t *nstream, const int64_t *offset,
                                       const double *stream_w0, double tfinal, double dt0,
                                       double rtol, double atol, int64_t nmax, double *out,
-                                      double *prog_final, int32_t *status, int64_t *stats,
+                                      double *body_final, int32_t *status, int64_t *stats,
                                       void *stream) {
-    if (!ext || !prog_w || !rel_t || !nstream || !offset || !prog_final || !status || dt0 == 0.0)
+    BodyTable bt;
+    if (make_body_table(bodies, nbody, bt) != GAMD_OK) return GAMD_ERR_INVALID;
+    if (!ext || nbody < 1 || !body_w || !rel_t || !nstream || !offset || !body_final ||
+        !status || dt0 == 0.0)
         return GAMD_ERR_INVALID;
     if (nmax == 0) nmax = 1000000;  // dop853.cpp:673-674
     else if (nmax < 0) return GAMD_ERR_INVALID;
@@ -810,7 +940,7 @@ extern "C" int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_pote
     a.nstiff = 1;          // dop853_step passes nstiff = 1 (dop853.pyx:72)
     a.uround = 2.3e-16;    // dop853_step passes uround = 0: dop853() picks 2.3e-16 (:733-734)
     StreamArgs sa;
-    sa.prog_w = prog_w;
+    sa.body_w = body_w;
     sa.rel_t = rel_t;
     sa.nstream = nstream;
     sa.offset = offset;
@@ -819,24 +949,82 @@ extern "C" int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_pote
     sa.tfinal = tfinal;
     sa.dt0 = dt0;
     sa.out = out;
-    sa.prog_final = prog_final;
+    sa.body_final = body_final;
     sa.status = status;
     sa.stats = stats;
-    const bool scf = ext->dev.has_scf || (prog && prog->dev.has_scf);
-    const bool mw = ext->dev.layout == LAYOUT_MW && !(prog && prog->dev.has_scf);
-    const DevPot *pp = prog ? prog->ddev : ext->ddev;  // not read without a progenitor
-    const int has_prog = prog ? 1 : 0;
+    const bool scf = ext->dev.has_scf || bt.scf;
+    const bool mw = ext->dev.layout == LAYOUT_MW && !bt.scf;
+    const int quads = nbody < 4 ? 1 : 0;  // a quad holds the bodies and at least one particle
     const int block = 256;
-    const unsigned grid = (unsigned)((4 * (int64_t)nrel + block - 1) / block);
     hipStream_t s = as_stream(stream);
+    if (quads) {
+        const unsigned grid = (unsigned)((4 * (int64_t)nrel + block - 1) / block);
+        if (scf)
+            hipLaunchKernelGGL((k_dop853_stream<true, LAYOUT_GENERIC, 4>), dim3(grid), dim3(block),
+                               0, s, ext->ddev, bt, quads, sa, a);
+        else if (mw)
+            hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_MW, 4>), dim3(grid), dim3(block), 0,
+                               s, ext->ddev, bt, quads, sa, a);
+        else
+            hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_GENERIC, 4>), dim3(grid),
+                               dim3(block), 0, s, ext->ddev, bt, quads, sa, a);
+        const int rc = launch_status();
+        if (rc != GAMD_OK) return rc;
+    }
+    // systems of more than 4 bodies (every system when the bodies alone fill a quad)
+    const unsigned grid = (unsigned)((64 * (int64_t)nrel + block - 1) / block);
     if (scf)
-        hipLaunchKernelGGL((k_dop853_stream<true, LAYOUT_GENERIC>), dim3(grid), dim3(block), 0, s,
-                           ext->ddev, pp, has_prog, sa, a);
+        hipLaunchKernelGGL((k_dop853_stream<true, LAYOUT_GENERIC, 64>), dim3(grid), dim3(block), 0,
+                           s, ext->ddev, bt, quads, sa, a);
     else if (mw)
-        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_MW>), dim3(grid), dim3(block), 0, s,
-                           ext->ddev, pp, has_prog, sa, a);
+        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_MW, 64>), dim3(grid), dim3(block), 0, s,
+                           ext->ddev, bt, quads, sa, a);
     else
-        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_GENERIC>), dim3(grid), dim3(block), 0, s,
-                           ext->ddev, pp, has_prog, sa, a);
+        hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_GENERIC, 64>), dim3(grid), dim3(block),
+                           0, s, ext->ddev, bt, quads, sa, a);
+    return launch_status();
+}
+
+extern "C" int gamd_nbody_dop853(const gamd_potential *ext, const gamd_potential *const *bodies,
+                                 int nbody, const double *w0, size_t n, double t0, double t1,
+                                 double h0, double rtol, double atol, int64_t nmax, int64_t nstiff,
+                                 double hmax, const double *t_out, size_t n_out, double *w_out,
+                                 double *w_final, int32_t *status, int64_t *stats, void *stream) {
+    BodyTable bt;
+    if (make_body_table(bodies, nbody, bt) != GAMD_OK) return GAMD_ERR_INVALID;
+    if (!ext || !w0 || !w_final || !status || (size_t)nbody > n || n > GAMD_MAX_BODIES ||
+        h0 == 0.0 || (w_out && (!t_out || n_out == 0)))
+        return GAMD_ERR_INVALID;
+    if (nmax == 0) nmax = 1000000;                   // dop853.cpp:673-674
+    else if (nmax < 0) return GAMD_ERR_INVALID;
+    if (nstiff == 0) nstiff = 1000;                  // :691-692
+    else if (nstiff < 0) nstiff = 0;                 // :693-694 (never tested)
+    if (n == 0) return GAMD_OK;
+    DopArgs a;
+    memset(&a, 0, sizeof(a));
+    a.w0 = w0;
+    a.N = (int64_t)n;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.nmax = nmax;
+    a.nstiff = nstiff;
+    a.hmax = hmax;
+    a.t_out = t_out;
+    a.n_out = w_out ? (int64_t)n_out : 0;
+    a.w_out = w_out;
+    a.w_final = w_final;
+    a.status = status;
+    a.stats = stats;
+    a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
+    const bool scf = ext->dev.has_scf || bt.scf;
+    const bool mw = ext->dev.layout == LAYOUT_MW && !bt.scf;
+    hipStream_t s = as_stream(stream);
+    auto kern = w_out ? (scf  ? k_dop853_nbody<true, true, LAYOUT_GENERIC>
+                         : mw ? k_dop853_nbody<false, true, LAYOUT_MW>
+                              : k_dop853_nbody<false, true, LAYOUT_GENERIC>)
+                      : (scf  ? k_dop853_nbody<true, false, LAYOUT_GENERIC>
+                         : mw ? k_dop853_nbody<false, false, LAYOUT_MW>
+                              : k_dop853_nbody<false, false, LAYOUT_GENERIC>);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, s, ext->ddev, bt, t0, t1, h0, a);
     return launch_status();
 }

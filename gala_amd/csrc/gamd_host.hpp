@@ -54,4 +54,24 @@ inline int cu_count() {
     return cus > 0 ? cus : 256;
 }
 
+// The body potentials of an N-body system, passed to kernels by value (the
+// kernel reads them from its argument segment with scalar loads).  p[j] is
+// body j's device table, nullptr for a massless body.
+struct BodyTable {
+    const DevPot *p[GAMD_MAX_MASSIVE];
+    int n;    // bodies 0..n-1 may carry a potential
+    int scf;  // some body potential has an SCF component
+};
+
+inline int make_body_table(const gamd_potential *const *bodies, int nbody, BodyTable &bt) {
+    if (nbody < 0 || nbody > GAMD_MAX_MASSIVE || (nbody > 0 && !bodies)) return GAMD_ERR_INVALID;
+    bt = BodyTable{};
+    bt.n = nbody;
+    for (int j = 0; j < nbody; j++) {
+        bt.p[j] = bodies[j] ? bodies[j]->ddev : nullptr;
+        if (bodies[j] && bodies[j]->dev.has_scf) bt.scf = 1;
+    }
+    return GAMD_OK;
+}
+
 }  // namespace gamd

@@ -1,22 +1,32 @@
-// gamd_mockstream.hip -- batched release-and-integrate of mock-stream particles.
+// gamd_mockstream.hip -- Leapfrog N-body systems and batched mock-stream
+// release-and-integrate.
 //
-// C ABI: gamd_mockstream_leapfrog (include/gala_amd.h).  Restates the
-// per-release loop of mockstream_leapfrog (dynamics/mockstream/mockstream.pyx:
-// 551-599) for one massive body, the progenitor, plus massless particles:
-//   * the release batch starts from the progenitor state on the forward grid
-//     (idx = int((t_i - t_0)/dt + 0.5), :546) and the particles' release
-//     states;
-//   * half kicks for every body (c_init_velocity_nbody, :571-578), then
-//     n_steps = int((tfinal - t_i)/dt + 0.5) leapfrog steps (:581-590) in
-//     which the progenitor moves first and each particle's kick adds the
-//     progenitor potential evaluated around the progenitor's drifted position
-//     (c_leapfrog_step_nbody / c_nbody_gradient_symplectic, leapfrog.pyx:
-//     140-158, cpotential.cpp:418-442);
-//   * the progenitor does not feel the particles (NullPotential bodies).
-// GPU structure: one lane per particle, and that lane carries its own copy of
-// the progenitor from the particle's release state, so the uneven per-release
-// lengths (0 .. nfull-1 steps) become independent lanes in one launch and each
-// lane reproduces the reference's per-release progenitor restart exactly.
+// C ABI: gamd_nbody_leapfrog, gamd_mockstream_leapfrog (include/gala_amd.h).
+//
+// gamd_nbody_leapfrog restates leapfrog_integrate_nbody (integrate/
+// cyintegrators/leapfrog.pyx:161-257): half kicks for every body
+// (c_init_velocity_nbody, :126-137), then per step every body in index order
+// drifts, evaluates the external gradient plus the potentials of the other
+// massive bodies centred on their current positions (c_nbody_gradient_
+// symplectic, potential/src/cpotential.cpp:418-442) and kicks
+// (c_leapfrog_step_nbody, :140-158).  Because the bodies move one after the
+// other, body i sees bodies below i already drifted and bodies above i not
+// yet; massless test particles (after the massive bodies) see them all
+// drifted and act on nothing.
+//
+// gamd_mockstream_leapfrog restates the per-release loop of
+// mockstream_leapfrog (dynamics/mockstream/mockstream.pyx:551-599): each
+// release batch starts from the bodies' states on the forward grid
+// (idx = int((t_i - t_0)/dt + 0.5), :546) and the particles' release states
+// and takes n_steps = int((tfinal - t_i)/dt + 0.5) steps (:581) of the same
+// leapfrog.
+//
+// GPU structure, both kernels: one lane per test particle, and every lane
+// carries its own copy of the massive bodies (at most GAMD_MAX_MASSIVE; up to
+// four stay in registers, more spill to scratch).  The bodies' arithmetic is
+// identical in every lane, so each lane reproduces the reference's bodies
+// exactly, and the uneven per-release lengths (0 .. nfull-1 steps) become
+// independent lanes of one launch.
 #include <string.h>
 
 #include "gamd_host.hpp"
@@ -25,94 +35,276 @@ using namespace gamd;
 
 namespace {
 
-// gradient of the progenitor potential at q, centred on xp, added to g
-template <bool SCF>
-__device__ __forceinline__ void prog_grad(const DevPot &PP, double x, double y, double z,
-                                          double px, double py, double pz, double &gx,
-                                          double &gy, double &gz) {
-    double fx, fy, fz;
-    body_grad<SCF>(PP, x, y, z, px, py, pz, fx, fy, fz);
-    gx += fx;
-    gy += fy;
-    gz += fz;
+// Gradient felt at (x, y, z) by body `self` (-1 for a test particle): the
+// external potential, then every other body with a potential, centred on that
+// body's position, in body order (c_gradient, then c_nbody_gradient_symplectic
+// adding each body's f2).
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void nbody_grad(const DevPot &E, const BodyTable &bt, int self,
+                                           const double (&bx)[KMAX][3], double x, double y,
+                                           double z, double &gx, double &gy, double &gz) {
+    pot_grad<SCF>(E, x, y, z, gx, gy, gz);
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j < bt.n && j != self && bt.p[j]) {
+            double fx, fy, fz;
+            body_grad<SCF>(*bt.p[j], x, y, z, bx[j][0], bx[j][1], bx[j][2], fx, fy, fz);
+            gx += fx;
+            gy += fy;
+            gz += fz;
+        }
+    }
 }
 
-template <bool SCF>
-__global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep, const DevPot *__restrict__ PPp, int has_prog,
+// The massive bodies of one lane: positions, half-step velocities and (when
+// the caller needs them) the synchronised velocities of the last step.
+template <int KMAX>
+struct Bodies {
+    double x[KMAX][3], vh[KMAX][3], v[KMAX][3];
+};
+
+// c_init_velocity_nbody for bodies 0..K-1 (positions unchanged, so the order
+// does not matter here)
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void bodies_init(const DevPot &E, const BodyTable &bt, int K,
+                                            double dt, Bodies<KMAX> &B) {
+#pragma unroll
+    for (int i = 0; i < KMAX; i++) {
+        if (i < K) {
+            double gx, gy, gz;
+            nbody_grad<SCF, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
+            B.vh[i][0] = B.v[i][0] - gx * dt / 2.0;
+            B.vh[i][1] = B.v[i][1] - gy * dt / 2.0;
+            B.vh[i][2] = B.v[i][2] - gz * dt / 2.0;
+        }
+    }
+}
+
+// one c_leapfrog_step_nbody for bodies 0..K-1, in index order
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void bodies_step(const DevPot &E, const BodyTable &bt, int K,
+                                            double dt, Bodies<KMAX> &B) {
+#pragma unroll
+    for (int i = 0; i < KMAX; i++) {
+        if (i < K) {
+            B.x[i][0] = B.x[i][0] + B.vh[i][0] * dt;
+            B.x[i][1] = B.x[i][1] + B.vh[i][1] * dt;
+            B.x[i][2] = B.x[i][2] + B.vh[i][2] * dt;
+            double gx, gy, gz;
+            nbody_grad<SCF, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
+            B.v[i][0] = B.vh[i][0] - gx * dt / 2.0;
+            B.v[i][1] = B.vh[i][1] - gy * dt / 2.0;
+            B.v[i][2] = B.vh[i][2] - gz * dt / 2.0;
+            B.vh[i][0] = B.vh[i][0] - gx * dt;
+            B.vh[i][1] = B.vh[i][1] - gy * dt;
+            B.vh[i][2] = B.vh[i][2] - gz * dt;
+        }
+    }
+}
+
+// a test particle's state and its init / step against the lane's bodies
+struct Particle {
+    double x, y, z, vx, vy, vz, hx, hy, hz;
+};
+
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void particle_init(const DevPot &E, const BodyTable &bt,
+                                              const Bodies<KMAX> &B, double dt, Particle &p) {
+    double gx, gy, gz;
+    nbody_grad<SCF, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
+    p.hx = p.vx - gx * dt / 2.0;
+    p.hy = p.vy - gy * dt / 2.0;
+    p.hz = p.vz - gz * dt / 2.0;
+}
+
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void particle_step(const DevPot &E, const BodyTable &bt,
+                                              const Bodies<KMAX> &B, double dt, Particle &p) {
+    p.x = p.x + p.hx * dt;
+    p.y = p.y + p.hy * dt;
+    p.z = p.z + p.hz * dt;
+    double gx, gy, gz;
+    nbody_grad<SCF, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
+    p.vx = p.hx - gx * dt / 2.0;
+    p.vy = p.hy - gy * dt / 2.0;
+    p.vz = p.hz - gz * dt / 2.0;
+    p.hx = p.hx - gx * dt;
+    p.hy = p.hy - gy * dt;
+    p.hz = p.hz - gz * dt;
+}
+
+// leapfrog_integrate_nbody: lane l integrates the K massive bodies and test
+// particle K + l (when there is one); lane 0 writes the bodies.
+template <bool SCF, bool SAVE, int KMAX>
+__global__ __launch_bounds__(64) void k_nbody_leapfrog(const DevPot *__restrict__ Ep, BodyTable bt,
+                                                       const double *__restrict__ w0, int64_t n,
+                                                       double dt, int64_t n_steps, int64_t stride,
+                                                       int64_t nsave, double *__restrict__ wout,
+                                                       double *__restrict__ wfinal) {
+    const DevPot &E = *Ep;
+    const int K = bt.n;
+    const int64_t ntest = n - K;
+    const int64_t lane = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (lane >= (ntest > 0 ? ntest : 1)) return;
+    const bool lead = lane == 0;
+    const bool has_p = ntest > 0;
+    const int64_t col = K + lane;
+    Bodies<KMAX> B;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            B.x[j][c] = j < K ? w0[c * n + j] : 0.0;
+            B.v[j][c] = j < K ? w0[(3 + c) * n + j] : 0.0;
+            B.vh[j][c] = 0.0;
+        }
+    }
+    Particle p{};
+    if (has_p) {
+        p.x = w0[col];
+        p.y = w0[n + col];
+        p.z = w0[2 * n + col];
+        p.vx = w0[3 * n + col];
+        p.vy = w0[4 * n + col];
+        p.vz = w0[5 * n + col];
+    }
+    auto store = [&](double *dst, int64_t so, int64_t row) {
+        // (6, rows, n) with row stride n: dst[(c * so + row) * n + col]
+        if (lead) {
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) {
+                if (j < K) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        dst[(c * so + row) * n + j] = B.x[j][c];
+                        dst[((3 + c) * so + row) * n + j] = B.v[j][c];
+                    }
+                }
+            }
+        }
+        if (has_p) {
+            dst[(0 * so + row) * n + col] = p.x;
+            dst[(1 * so + row) * n + col] = p.y;
+            dst[(2 * so + row) * n + col] = p.z;
+            dst[(3 * so + row) * n + col] = p.vx;
+            dst[(4 * so + row) * n + col] = p.vy;
+            dst[(5 * so + row) * n + col] = p.vz;
+        }
+    };
+    if (SAVE) store(wout, nsave, 0);
+    bodies_init<SCF, KMAX>(E, bt, K, dt, B);
+    if (has_p) particle_init<SCF, KMAX>(E, bt, B, dt, p);
+    for (int64_t s = 1; s <= n_steps; s++) {
+        bodies_step<SCF, KMAX>(E, bt, K, dt, B);
+        if (has_p) particle_step<SCF, KMAX>(E, bt, B, dt, p);
+        if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
+    }
+    if (wfinal) store(wfinal, 1, 0);
+}
+
+// mockstream_leapfrog's release batches: lane i = particle i with its own
+// copy of the bodies from their states at its release.
+template <bool SCF, int KMAX>
+__global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep, BodyTable bt,
                                                    const double *__restrict__ traj, int64_t nfull,
                                                    double t0, double dt,
                                                    const double *__restrict__ w0,
                                                    const double *__restrict__ rel_t, int64_t n,
                                                    double tfinal, double *__restrict__ out) {
     const DevPot &E = *Ep;
-    const DevPot &PP = *PPp;
+    const int K = bt.n;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t idx = (int64_t)((rel_t[i] - t0) / dt + 0.5);
     if (idx < 0) idx = 0;
     if (idx >= nfull) idx = nfull - 1;
     const int64_t nsteps = (int64_t)((tfinal - rel_t[i]) / dt + 0.5);
-    // progenitor at release
-    double px = traj[idx], py = traj[nfull + idx], pz = traj[2 * nfull + idx];
-    double pvx = traj[3 * nfull + idx], pvy = traj[4 * nfull + idx], pvz = traj[5 * nfull + idx];
-    // particle
-    double x = w0[i], y = w0[n + i], z = w0[2 * n + i];
-    double vx = w0[3 * n + i], vy = w0[4 * n + i], vz = w0[5 * n + i];
-    double gx, gy, gz;
-    // c_init_velocity_nbody: progenitor (body 0), then the particle
-    pot_grad<SCF>(E, px, py, pz, gx, gy, gz);
-    double phx = pvx - gx * dt / 2.0, phy = pvy - gy * dt / 2.0, phz = pvz - gz * dt / 2.0;
-    pot_grad<SCF>(E, x, y, z, gx, gy, gz);
-    if (has_prog) prog_grad<SCF>(PP, x, y, z, px, py, pz, gx, gy, gz);
-    double hx = vx - gx * dt / 2.0, hy = vy - gy * dt / 2.0, hz = vz - gz * dt / 2.0;
-    for (int64_t s = 0; s < nsteps; s++) {
-        px = px + phx * dt;
-        py = py + phy * dt;
-        pz = pz + phz * dt;
-        pot_grad<SCF>(E, px, py, pz, gx, gy, gz);
-        phx = phx - gx * dt;
-        phy = phy - gy * dt;
-        phz = phz - gz * dt;
-        x = x + hx * dt;
-        y = y + hy * dt;
-        z = z + hz * dt;
-        pot_grad<SCF>(E, x, y, z, gx, gy, gz);
-        if (has_prog) prog_grad<SCF>(PP, x, y, z, px, py, pz, gx, gy, gz);
-        vx = hx - gx * dt / 2.0;
-        vy = hy - gy * dt / 2.0;
-        vz = hz - gz * dt / 2.0;
-        hx = hx - gx * dt;
-        hy = hy - gy * dt;
-        hz = hz - gz * dt;
+    // bodies at release: traj (6, nfull, K)
+    Bodies<KMAX> B;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            B.x[j][c] = j < K ? traj[(c * nfull + idx) * K + j] : 0.0;
+            B.v[j][c] = j < K ? traj[((3 + c) * nfull + idx) * K + j] : 0.0;
+            B.vh[j][c] = 0.0;
+        }
     }
-    out[i] = x;
-    out[n + i] = y;
-    out[2 * n + i] = z;
-    out[3 * n + i] = vx;
-    out[4 * n + i] = vy;
-    out[5 * n + i] = vz;
+    Particle p{};
+    p.x = w0[i];
+    p.y = w0[n + i];
+    p.z = w0[2 * n + i];
+    p.vx = w0[3 * n + i];
+    p.vy = w0[4 * n + i];
+    p.vz = w0[5 * n + i];
+    // c_init_velocity_nbody: the bodies, then the particle
+    bodies_init<SCF, KMAX>(E, bt, K, dt, B);
+    particle_init<SCF, KMAX>(E, bt, B, dt, p);
+    for (int64_t s = 0; s < nsteps; s++) {
+        bodies_step<SCF, KMAX>(E, bt, K, dt, B);
+        particle_step<SCF, KMAX>(E, bt, B, dt, p);
+    }
+    out[i] = p.x;
+    out[n + i] = p.y;
+    out[2 * n + i] = p.z;
+    out[3 * n + i] = p.vx;
+    out[4 * n + i] = p.vy;
+    out[5 * n + i] = p.vz;
 }
 
 }  // namespace
 
-extern "C" int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *prog,
-                                        const double *prog_traj, size_t nfull, double t0, double dt,
-                                        const double *w0, const double *rel_t, size_t n,
-                                        double tfinal, double *out, void *stream) {
-    if (!ext || !prog_traj || nfull == 0 || !w0 || !rel_t || !out || dt == 0.0)
+extern "C" int gamd_nbody_leapfrog(const gamd_potential *ext, const gamd_potential *const *bodies,
+                                   int nbody, const double *w0, size_t n, double t0, double dt,
+                                   int64_t n_steps, int64_t save_stride, double *w_out,
+                                   double *w_final, void *stream) {
+    (void)t0;  // static potentials only: the time argument of c_gradient is unused
+    BodyTable bt;
+    if (make_body_table(bodies, nbody, bt) != GAMD_OK) return GAMD_ERR_INVALID;
+    if (!ext || !w0 || n_steps < 0 || (w_out && save_stride <= 0) || (!w_out && !w_final) ||
+        (size_t)nbody > n)
         return GAMD_ERR_INVALID;
     if (n == 0) return GAMD_OK;
-    const bool scf = ext->dev.has_scf || (prog && prog->dev.has_scf);
-    const DevPot *pp = prog ? prog->ddev : ext->ddev;  // not read when there is no progenitor
-    const int has_prog = prog ? 1 : 0;
+    const int64_t nsave = w_out ? n_steps / save_stride + 1 : 0;
+    const int64_t lanes = (int64_t)n > nbody ? (int64_t)n - nbody : 1;
     const unsigned block = 64;
-    if (scf)
-        hipLaunchKernelGGL(k_mockstream<true>, dim3(grid_for(n, block)), dim3(block), 0,
-                           as_stream(stream), ext->ddev, pp, has_prog, prog_traj, (int64_t)nfull, t0, dt, w0,
-                           rel_t, (int64_t)n, tfinal, out);
-    else
-        hipLaunchKernelGGL(k_mockstream<false>, dim3(grid_for(n, block)), dim3(block), 0,
-                           as_stream(stream), ext->ddev, pp, has_prog, prog_traj, (int64_t)nfull, t0, dt, w0,
-                           rel_t, (int64_t)n, tfinal, out);
+    const bool scf = ext->dev.has_scf || bt.scf;
+    // bodies per lane: 1, up to 4 (registers), up to 8 (the body arrays spill to scratch)
+    auto pick = [&](auto k1, auto k4, auto k8) { return nbody <= 1 ? k1 : nbody <= 4 ? k4 : k8; };
+    auto kern = w_out ? (scf ? pick(k_nbody_leapfrog<true, true, 1>, k_nbody_leapfrog<true, true, 4>,
+                                    k_nbody_leapfrog<true, true, 8>)
+                             : pick(k_nbody_leapfrog<false, true, 1>, k_nbody_leapfrog<false, true, 4>,
+                                    k_nbody_leapfrog<false, true, 8>))
+                      : (scf ? pick(k_nbody_leapfrog<true, false, 1>, k_nbody_leapfrog<true, false, 4>,
+                                    k_nbody_leapfrog<true, false, 8>)
+                             : pick(k_nbody_leapfrog<false, false, 1>, k_nbody_leapfrog<false, false, 4>,
+                                    k_nbody_leapfrog<false, false, 8>));
+    hipLaunchKernelGGL(kern, dim3(grid_for((size_t)lanes, block)), dim3(block), 0, as_stream(stream),
+                       ext->ddev, bt, w0, (int64_t)n, dt, n_steps, w_out ? save_stride : 1, nsave,
+                       w_out, w_final);
+    return launch_status();
+}
+
+extern "C" int gamd_mockstream_leapfrog(const gamd_potential *ext,
+                                        const gamd_potential *const *bodies, int nbody,
+                                        const double *body_traj, size_t nfull, double t0,
+                                        double dt, const double *w0, const double *rel_t, size_t n,
+                                        double tfinal, double *out, void *stream) {
+    BodyTable bt;
+    if (make_body_table(bodies, nbody, bt) != GAMD_OK) return GAMD_ERR_INVALID;
+    if (!ext || nbody < 1 || !body_traj || nfull == 0 || !w0 || !rel_t || !out || dt == 0.0)
+        return GAMD_ERR_INVALID;
+    if (n == 0) return GAMD_OK;
+    const bool scf = ext->dev.has_scf || bt.scf;
+    auto kern = scf ? (nbody == 1   ? k_mockstream<true, 1>
+                       : nbody <= 4 ? k_mockstream<true, 4>
+                                    : k_mockstream<true, 8>)
+                    : (nbody == 1   ? k_mockstream<false, 1>
+                       : nbody <= 4 ? k_mockstream<false, 4>
+                                    : k_mockstream<false, 8>);
+    const unsigned block = 64;
+    hipLaunchKernelGGL(kern, dim3(grid_for(n, block)), dim3(block), 0, as_stream(stream),
+                       ext->ddev, bt, body_traj, (int64_t)nfull, t0, dt, w0, rel_t, (int64_t)n,
+                       tfinal, out);
     return launch_status();
 }

@@ -289,13 +289,16 @@ class StreaklineStreamDF(BaseStreamDF):
 # ---------------------------------------------------------------------------
 
 class DirectNBody:
-    """Massive bodies in an external potential (dynamics/nbody/core.py).  Only
-    the case the stream generator builds is supported: one massive body (the
-    progenitor), whose own potential does not act on itself, so its orbit is a
-    test-particle orbit in the external potential: Leapfrog
-    (leapfrog.pyx:126-257 with nbody_i == 0 skipped, cpotential.cpp:430) or,
-    gala's default, DOP853 (direct_nbody_dop853, nbody.pyx:30-115: dense
-    output on t, h0 = t[1] - t[0], stiffness test off)."""
+    """Bodies with their own potentials in an external potential
+    (dynamics/nbody/core.py:22-283).  particle_potentials[i] is body i's
+    potential, centred on body i (None: a massless test particle).
+    integrate_orbit puts the massive bodies first, as the reference does
+    (core.py:212-227), and returns the orbits in that order.  Integrators:
+    DOP853 (the default; direct_nbody_dop853, nbody.pyx:30-115: all bodies one
+    system with a shared adaptive step, dense output on t, h0 = t[1] - t[0],
+    stiffness test off) and Leapfrog (leapfrog_integrate_nbody,
+    leapfrog.pyx:161-257).  Limits: at most 8 massive bodies; DOP853 takes at
+    most 64 bodies in all."""
 
     def __init__(self, w0, particle_potentials, external_potential=None, frame=None, units=None,
                  save_all=True):
@@ -303,44 +306,59 @@ class DirectNBody:
             w = w0.w()
         else:
             w = np.asarray(w0, dtype=np.float64)
-        self.w0 = w.reshape(6, -1)
-        if self.w0.shape[1] != 1:
-            raise NotImplementedError("only a single massive body (the progenitor) is supported")
+        self.w0 = np.ascontiguousarray(w.reshape(6, -1))
+        if self.w0.shape[1] != len(particle_potentials):
+            raise ValueError("The number of initial conditions in `w0` must match the number of "
+                             "particle potentials passed in with `particle_potentials`.")
         self.particle_potentials = [NullPotential() if p is None else p
                                     for p in particle_potentials]
+        if external_potential is None:
+            external_potential = NullPotential()
         self.external_potential = external_potential
         self.H = Hamiltonian(external_potential, frame)
+        self.frame = self.H.frame
         self.units = self.H.units
         self.save_all = save_all
 
+    def _reorganized(self):
+        """(w0 with massive bodies first, their potentials) (core.py:212-227)."""
+        massive = [i for i, p in enumerate(self.particle_potentials)
+                   if not isinstance(p, NullPotential)]
+        test = [i for i, p in enumerate(self.particle_potentials) if isinstance(p, NullPotential)]
+        if len(massive) > _lib.MAX_MASSIVE:
+            raise NotImplementedError(f"at most {_lib.MAX_MASSIVE} massive bodies are supported")
+        w0 = np.ascontiguousarray(self.w0[:, massive + test])
+        return w0, [self.particle_potentials[i] for i in massive]
+
     def integrate_orbit(self, Integrator=None, Integrator_kwargs=None, **time_spec):
+        from .integrate import nbody_dop853, nbody_leapfrog
         Integrator = get_integrator(Integrator or DOPRI853Integrator)
         t = parse_time_specification(self.units, **time_spec)
+        w0, bodies = self._reorganized()
         if Integrator is DOPRI853Integrator:
-            from .integrate import dop853
+            if w0.shape[1] > _lib.MAX_BODIES:
+                raise NotImplementedError(f"DOP853 N-body systems hold at most {_lib.MAX_BODIES} "
+                                          "bodies")
             kw = dict(Integrator_kwargs or {})
-            res = dop853(self.external_potential, self.w0, t[0], t[-1], t[1] - t[0],
-                         rtol=float(kw.get("rtol", 1e-10)), atol=float(kw.get("atol", 1e-10)),
-                         nmax=int(kw.get("nmax", 0)), nstiff=-1,
-                         hmax=float(kw.get("dt_max", 0.0)),
-                         t_out=t if self.save_all else None)
+            res = nbody_dop853(self.external_potential, bodies, w0, t[0], t[-1], t[1] - t[0],
+                               rtol=float(kw.get("rtol", 1e-10)), atol=float(kw.get("atol", 1e-10)),
+                               nmax=int(kw.get("nmax", 0)), nstiff=-1,
+                               hmax=float(kw.get("dt_max", 0.0)),
+                               t_out=t if self.save_all else None)
             st = dev.to_host(res["status"])
             if int(kw.get("err_if_fail", 1)) == 1 and np.any(st < 0):
-                raise RuntimeError(f"Integration failed with code {int(st[st < 0][0])}")
-            if self.save_all:
-                w = dev.to_host(res["w_out"])  # (6, ntimes, 1)
-                return Orbit(pos=w[:3], vel=w[3:], t=t, hamiltonian=self.H)
-            w = dev.to_host(res["w_final"])
-            return PhaseSpacePosition(w[:3], w[3:])
-        if Integrator is not LeapfrogIntegrator:
+                raise RuntimeError(f"Integration failed with code {int(st[0])}")
+            wout, wfin = res.get("w_out"), res["w_final"]
+        elif Integrator is LeapfrogIntegrator:
+            wout, wfin = nbody_leapfrog(self.external_potential, bodies, w0, t[1] - t[0],
+                                        len(t) - 1, save=self.save_all)
+        else:
             raise NotImplementedError("N-body integration supports Leapfrog and DOP853")
-        wout, wfin = leapfrog(self.external_potential, self.w0, t[1] - t[0], len(t) - 1,
-                              save=self.save_all)
         if self.save_all:
-            w = dev.to_host(wout)  # (6, ntimes, 1)
+            w = dev.to_host(wout)  # (6, ntimes, n)
             return Orbit(pos=w[:3], vel=w[3:], t=t, hamiltonian=self.H)
         w = dev.to_host(wfin)
-        return PhaseSpacePosition(w[:3], w[3:])
+        return PhaseSpacePosition(w[:3], w[3:], frame=self.frame)
 
 
 class MockStreamGenerator:
@@ -361,12 +379,36 @@ class MockStreamGenerator:
         self.self_gravity = progenitor_potential is not None
         self.progenitor_potential = progenitor_potential
 
+    def _system(self, pw0, nbody):
+        """The bodies integrated with the stream (_get_nbody,
+        mockstream_generator.py:79-117): the progenitor (body 0, potential
+        progenitor_potential) and the perturbers of `nbody`.  Returns (w0 (6, K),
+        body potentials (None = massless))."""
+        prog_pot = self.progenitor_potential
+        if nbody is None:
+            return pw0, [prog_pot]
+        if nbody.external_potential != self.hamiltonian.potential:
+            raise ValueError("The external potential of the input nbody instance must match the "
+                             "potential of the mock stream input hamiltonian! "
+                             f"{nbody.external_potential} vs. {self.hamiltonian.potential}")
+        pots = [prog_pot] + [None if isinstance(p, NullPotential) else p
+                             for p in nbody.particle_potentials]
+        if any(p is None for p in pots):
+            # DirectNBody.integrate_orbit would move massless bodies behind the
+            # massive ones while the release systems keep the original order
+            raise NotImplementedError("with perturbers, the progenitor and every perturber must "
+                                      "have a potential")
+        if len(pots) > _lib.MAX_MASSIVE:
+            raise NotImplementedError(f"at most {_lib.MAX_MASSIVE} bodies (progenitor and "
+                                      "perturbers) are supported")
+        return np.ascontiguousarray(np.hstack([pw0, nbody.w0])), pots
+
     def run(self, prog_w0, prog_mass, nbody=None, release_every=1, n_particles=1,
             output_every=None, output_filename=None, check_filesize=True, overwrite=False,
             progress=False, Integrator=None, Integrator_kwargs=None, **time_spec):
-        """Returns (stream MockStream, progenitor PhaseSpacePosition)."""
-        if nbody is not None:
-            raise NotImplementedError("extra massive perturbers (nbody=) are not supported")
+        """Returns (stream MockStream, bodies PhaseSpacePosition (3, K): the
+        progenitor and any perturbers at the final time)."""
+        from .integrate import nbody_leapfrog
         if output_every is not None:
             raise NotImplementedError("snapshot output (output_every=) is not supported")
         Integrator = get_integrator(Integrator or DOPRI853Integrator)
@@ -380,17 +422,21 @@ class MockStreamGenerator:
             pw0 = prog_w0.w().reshape(6, 1)
         else:
             pw0 = np.asarray(prog_w0, dtype=np.float64).reshape(6, 1)
+        sys_w0, bodies = self._system(pw0, nbody)
+        K = len(bodies)
         if Integrator is DOPRI853Integrator:
-            return self._run_dop853(pw0, prog_mass, t, release_every, n_particles,
+            return self._run_dop853(sys_w0, bodies, prog_mass, t, release_every, n_particles,
                                     dict(Integrator_kwargs or {}))
-        # progenitor orbit over t (backward if dt < 0), then reversed
-        wout, _ = leapfrog(pot, pw0, t[1] - t[0], len(t) - 1, save=True)
-        w = dev.to_host(wout)[:, :, 0]
+        # the bodies' orbits over t (backward if dt < 0), then reversed
+        # (DirectNBody.integrate_orbit -> leapfrog_integrate_nbody)
+        wout, _ = nbody_leapfrog(pot, bodies, sys_w0, t[1] - t[0], len(t) - 1, save=True)
+        w = dev.to_host(wout)  # (6, ntimes, K)
         orbit_t = t
         if t[1] < t[0]:
             w = w[:, ::-1]
             orbit_t = t[::-1]
-        prog_orbit = Orbit(pos=w[:3], vel=w[3:], t=orbit_t, hamiltonian=self.hamiltonian)
+        prog_orbit = Orbit(pos=w[:3, :, 0], vel=w[3:, :, 0], t=orbit_t,
+                           hamiltonian=self.hamiltonian)
         stream_w0 = self.df.sample(prog_orbit, prog_mass, hamiltonian=self.hamiltonian,
                                    release_every=release_every, n_particles=n_particles)
         # release bookkeeping (mockstream_generator.py:272-283)
@@ -398,57 +444,58 @@ class MockStreamGenerator:
         all_nstream = np.zeros(len(orbit_t), dtype=int)
         for t1, n in zip(unq_t1s, nstream):
             all_nstream[np.isclose(orbit_t, t1)] = n
-        # the forward N-body trajectory of the progenitor (mockstream.pyx:536-549)
+        # the bodies' forward trajectory on the full grid (mockstream.pyx:536-549)
         dt = orbit_t[1] - orbit_t[0]
         nfull = len(orbit_t)
-        traj_dev, _ = leapfrog(pot, np.ascontiguousarray(w[:, 0:1]), dt, nfull - 1, save=True)
-        traj_dev = traj_dev[:, :, 0].contiguous()
+        traj_dev, _ = nbody_leapfrog(pot, bodies, np.ascontiguousarray(w[:, 0, :]), dt,
+                                     nfull - 1, save=True)  # (6, nfull, K)
         w0s = np.ascontiguousarray(stream_w0.w())
         n = w0s.shape[1]
         rel_t = np.asarray(stream_w0.release_time, dtype=np.float64)
         out = dev.empty((6, n))
         lib = _lib.load()
-        prog_handle = self.progenitor_potential.c_handle if self.self_gravity else None
         w0d = dev.to_dev(w0s)
         reld = dev.to_dev(rel_t)
         tfinal = float(orbit_t[-1])
-        _lib.check(lib.gamd_mockstream_leapfrog(pot.c_handle, prog_handle, traj_dev.data_ptr(),
+        handles = _lib.body_array([None if b is None else b.c_handle for b in bodies])
+        _lib.check(lib.gamd_mockstream_leapfrog(pot.c_handle, handles, K, traj_dev.data_ptr(),
                                                 nfull, float(orbit_t[0]), float(dt),
                                                 w0d.data_ptr(), reld.data_ptr(), n, tfinal,
                                                 out.data_ptr(), dev.stream_ptr()),
                    "gamd_mockstream_leapfrog")
         raw_stream = dev.to_host(out)
         traj = dev.to_host(traj_dev)
-        # the progenitor state the reference returns: that of the last release
-        # batch, integrated from its release state to tfinal
+        # the bodies the reference returns: those of the last release batch,
+        # integrated from their release states to tfinal
         nz = np.nonzero(all_nstream)[0]
         last_idx = int(nz[-1]) if len(nz) else nfull - 1
         if last_idx == nfull - 1:
-            prog_final = traj[:, -1]
+            body_final = traj[:, -1, :]
         else:
-            _, pf = leapfrog(pot, np.ascontiguousarray(traj[:, last_idx:last_idx + 1]), dt,
-                             nfull - 1 - last_idx, save=False)
-            prog_final = dev.to_host(pf)[:, 0]
+            _, bf = nbody_leapfrog(pot, bodies, np.ascontiguousarray(traj[:, last_idx, :]), dt,
+                                   nfull - 1 - last_idx, save=False)
+            body_final = dev.to_host(bf)
         stream = MockStream(pos=raw_stream[:3], vel=raw_stream[3:],
                             release_time=stream_w0.release_time,
                             lead_trail=stream_w0.lead_trail, frame=self.hamiltonian.frame)
-        prog = PhaseSpacePosition(pos=prog_final[:3, None], vel=prog_final[3:, None],
-                                  frame=self.hamiltonian.frame)
-        return stream, prog
+        nbody_w = PhaseSpacePosition(pos=body_final[:3], vel=body_final[3:],
+                                     frame=self.hamiltonian.frame)
+        return stream, nbody_w
 
-    def _run_dop853(self, pw0, prog_mass, t, release_every, n_particles, kw):
+    def _run_dop853(self, sys_w0, bodies, prog_mass, t, release_every, n_particles, kw):
         """The DOP853 path of run() (mockstream_generator.py:229-330 with
         mockstream_dop853, mockstream.pyx:176-303):
-          1. the progenitor's orbit over t by DOP853 with dense output at t
-             (DirectNBody.integrate_orbit -> direct_nbody_dop853, stiffness
-             test off), reversed when t runs backwards;
-          2. the DF samples particles along it;
-          3. the progenitor again, forward from the earliest time, densely
-             output at the release times (dop853_helper in mockstream_dop853);
-          4. every release's progenitor + particles system by DOP853 to the
-             final time (gamd_mockstream_dop853)."""
-        from .integrate import dop853
+          1. the bodies' orbits over t by DOP853 as one system with dense
+             output at t (DirectNBody.integrate_orbit -> direct_nbody_dop853,
+             stiffness test off), reversed when t runs backwards;
+          2. the DF samples particles along the progenitor's (body 0's) orbit;
+          3. the bodies again, forward from the earliest time, densely output
+             at the release times (dop853_helper in mockstream_dop853);
+          4. every release's bodies + particles system by DOP853 to the final
+             time (gamd_mockstream_dop853)."""
+        from .integrate import nbody_dop853
         pot = self.hamiltonian.potential
+        K = len(bodies)
         atol = float(kw.get("atol", 1e-10))
         rtol = float(kw.get("rtol", 1e-10))
         nmax = int(kw.get("nmax", 0))
@@ -462,14 +509,16 @@ class MockStreamGenerator:
                 raise RuntimeError(f"Integration failed with code {int(st[st < 0][0])} ({what})")
             return res
 
-        res = checked(dop853(pot, pw0, t[0], t[-1], t[1] - t[0], rtol=rtol, atol=atol, nmax=nmax,
-                             nstiff=-1, hmax=dt_max, t_out=t), "progenitor orbit")
-        w = dev.to_host(res["w_out"])[:, :, 0]
+        res = checked(nbody_dop853(pot, bodies, sys_w0, t[0], t[-1], t[1] - t[0], rtol=rtol,
+                                   atol=atol, nmax=nmax, nstiff=-1, hmax=dt_max, t_out=t),
+                      "bodies' orbits")
+        w = dev.to_host(res["w_out"])  # (6, ntimes, K)
         orbit_t = t
         if t[1] < t[0]:
             w = w[:, ::-1]
             orbit_t = t[::-1]
-        prog_orbit = Orbit(pos=w[:3], vel=w[3:], t=orbit_t, hamiltonian=self.hamiltonian)
+        prog_orbit = Orbit(pos=w[:3, :, 0], vel=w[3:, :, 0], t=orbit_t,
+                           hamiltonian=self.hamiltonian)
         stream_w0 = self.df.sample(prog_orbit, prog_mass, hamiltonian=self.hamiltonian,
                                    release_every=release_every, n_particles=n_particles)
         # release bookkeeping (mockstream_generator.py:272-292)
@@ -485,42 +534,44 @@ class MockStreamGenerator:
         ns = all_nstream[nstream_idx].astype(np.int32)
         if len(time) < 2:
             raise ValueError("a DOP853 mock stream needs at least two release times")
-        if ns.max() > 3:
-            raise NotImplementedError("at most 3 particles per release time "
-                                      "(n_particles <= 1 per tail)")
-        # the progenitor, forward, at the release times (mockstream.pyx:274-282)
-        res2 = checked(dop853(pot, np.ascontiguousarray(w[:, 0:1]), time[0], time[-1],
-                              time[1] - time[0], rtol=rtol, atol=atol, nmax=nmax,
-                              nstiff=nstiff, hmax=dt_max, t_out=time), "progenitor at releases")
-        prog_w = dev.to_dev(np.ascontiguousarray(dev.to_host(res2["w_out"])[:, :, 0].T))
+        if K + ns.max() > _lib.MAX_BODIES:
+            raise NotImplementedError(f"a release system holds at most {_lib.MAX_BODIES} bodies "
+                                      "(progenitor, perturbers and the release's particles)")
+        # the bodies, forward, at the release times (mockstream.pyx:274-282)
+        res2 = checked(nbody_dop853(pot, bodies, np.ascontiguousarray(w[:, 0, :]), time[0],
+                                    time[-1], time[1] - time[0], rtol=rtol, atol=atol, nmax=nmax,
+                                    nstiff=nstiff, hmax=dt_max, t_out=time),
+                       "bodies at releases")
+        body_w = dev.to_dev(np.ascontiguousarray(
+            dev.to_host(res2["w_out"]).transpose(1, 2, 0)))  # (nrel, K, 6)
         nrel = len(time)
         offsets = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
         sw = np.ascontiguousarray(stream_w0.w().T)
         ntot = sw.shape[0]
         t_ = dev.torch()
         out = t_.empty((max(ntot, 1), 6), dtype=t_.float64, device="cuda")
-        prog_final = t_.empty((nrel, 6), dtype=t_.float64, device="cuda")
+        body_final = t_.empty((nrel, K, 6), dtype=t_.float64, device="cuda")
         status = t_.empty(nrel, dtype=t_.int32, device="cuda")
         swd = dev.to_dev(sw)
         rel_d = dev.to_dev(np.ascontiguousarray(unq_t1s, dtype=np.float64))
         ns_d = dev.to_dev(ns, dtype=t_.int32)
         off_d = dev.to_dev(offsets, dtype=t_.int64)
         lib = _lib.load()
-        prog_handle = self.progenitor_potential.c_handle if self.self_gravity else None
-        _lib.check(lib.gamd_mockstream_dop853(pot.c_handle, prog_handle, prog_w.data_ptr(), nrel,
+        handles = _lib.body_array([None if b is None else b.c_handle for b in bodies])
+        _lib.check(lib.gamd_mockstream_dop853(pot.c_handle, handles, K, body_w.data_ptr(), nrel,
                                               rel_d.data_ptr(), ns_d.data_ptr(), off_d.data_ptr(),
                                               swd.data_ptr(), float(orbit_t[-1]),
                                               float(time[1] - time[0]), rtol, atol, nmax,
-                                              out.data_ptr(), prog_final.data_ptr(),
+                                              out.data_ptr(), body_final.data_ptr(),
                                               status.data_ptr(), None, dev.stream_ptr()),
                    "gamd_mockstream_dop853")
         checked({"status": status}, "release systems")
         raw_stream = dev.to_host(out)[:ntot].T
         last = int(np.nonzero(ns)[0][-1])
-        pf = dev.to_host(prog_final)[last]
+        bf = dev.to_host(body_final)[last]  # (K, 6)
         stream = MockStream(pos=raw_stream[:3], vel=raw_stream[3:],
                             release_time=stream_w0.release_time,
                             lead_trail=stream_w0.lead_trail, frame=self.hamiltonian.frame)
-        prog = PhaseSpacePosition(pos=pf[:3, None], vel=pf[3:, None],
-                                  frame=self.hamiltonian.frame)
-        return stream, prog
+        nbody_w = PhaseSpacePosition(pos=bf[:, :3].T, vel=bf[:, 3:].T,
+                                     frame=self.hamiltonian.frame)
+        return stream, nbody_w

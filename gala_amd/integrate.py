@@ -14,7 +14,7 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
-           "integrate_hamiltonian"]
+           "nbody_leapfrog", "nbody_dop853", "integrate_hamiltonian"]
 
 
 class Integrator:
@@ -172,6 +172,59 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
                                wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
                                dev.ptr(ordd), int(n_tail), int(block), dev.stream_ptr()),
                "gamd_dop853")
+    out = {"w_final": wfin, "status": status, "stats": stats}
+    if wout is not None:
+        out["w_out"] = wout
+    return out
+
+
+def _bodies(body_potentials):
+    """Host array of body handles (None = massless) and the keepalive list."""
+    handles = [None if p is None else p.c_handle for p in body_potentials]
+    return _lib.body_array(handles), len(handles)
+
+
+def nbody_leapfrog(potential, body_potentials, w0, dt, n_steps, save_stride=1, save=True,
+                   t0=0.0):
+    """leapfrog_integrate_nbody on device (leapfrog.pyx:161-257).
+    body_potentials: potentials of bodies 0..K-1 (None = massless), each
+    centred on its body; bodies K.. of w0 (6, n) are test particles.
+    Returns (w_out (6, n_steps//save_stride + 1, n) or None, w_final (6, n))."""
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    n = w0.shape[1]
+    arr, K = _bodies(body_potentials)
+    wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, n)) if save else None
+    wfin = dev.empty((6, n))
+    _lib.check(lib.gamd_nbody_leapfrog(potential.c_handle, arr, K, w0.data_ptr(), n, float(t0),
+                                       float(dt), int(n_steps), int(save_stride), dev.ptr(wout),
+                                       wfin.data_ptr(), dev.stream_ptr()), "gamd_nbody_leapfrog")
+    return wout, wfin
+
+
+def nbody_dop853(potential, body_potentials, w0, t0, t1, h0, rtol=1e-10, atol=1e-10, nmax=0,
+                 nstiff=-1, hmax=0.0, t_out=None):
+    """direct_nbody_dop853 on device (nbody.pyx:30-115): the n bodies of w0
+    (6, n) as one system with one shared adaptive step.
+    Returns dict(w_final (6, n), status (1,), stats (1, 4)[, w_out (6, n_out, n)])."""
+    t = dev.require_gpu()
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    n = w0.shape[1]
+    arr, K = _bodies(body_potentials)
+    wfin = dev.empty((6, n))
+    status = dev.empty((1,), dtype=t.int32)
+    stats = dev.empty((1, 4), dtype=t.int64)
+    wout = tout = None
+    if t_out is not None:
+        tout = dev.to_dev(t_out)
+        wout = dev.empty((6, tout.shape[0], n))
+    _lib.check(lib.gamd_nbody_dop853(potential.c_handle, arr, K, w0.data_ptr(), n, float(t0),
+                                     float(t1), float(h0), float(rtol), float(atol), int(nmax),
+                                     int(nstiff), float(hmax), dev.ptr(tout),
+                                     0 if tout is None else tout.shape[0], dev.ptr(wout),
+                                     wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
+                                     dev.stream_ptr()), "gamd_nbody_dop853")
     out = {"w_final": wfin, "status": status, "stats": stats}
     if wout is not None:
         out["w_out"] = wout

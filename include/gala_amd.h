@@ -142,41 +142,81 @@ int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const dou
                 double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
                 int64_t n_tail, int block, void *stream);
 
+/* ---- N-body systems ------------------------------------------------------ */
+#define GAMD_MAX_BODIES 64   /* bodies of one DOP853 N-body system (one wave) */
+#define GAMD_MAX_MASSIVE 8   /* bodies carried by every lane of the Leapfrog N-body kernels */
+
+/* Body potentials: `bodies` is a HOST array of nbody potential handles for
+ * bodies 0..nbody-1; body j's potential acts on every other body, centred on
+ * body j's position (c_nbody_acceleration / c_nbody_gradient_symplectic set
+ * do_shift_rotate = 1 and q0 = the body's position, keeping each
+ * component's R, cpotential.cpp:389-442).  A NULL entry is a massless body
+ * (NullPotential, skipped as the reference skips pots[j]->null).  Bodies
+ * from index nbody on are massless test particles (DirectNBody puts the
+ * massive bodies first, dynamics/nbody/core.py:212-227). */
+
+/* Replaces leapfrog_integrate_nbody (leapfrog.pyx:161-257): n bodies, w0
+ * (6, n), fixed dt, n_steps steps.  Within a step bodies move in index order
+ * (c_leapfrog_step_nbody, leapfrog.pyx:140-158, :241-252): body i's kick sees
+ * bodies below i already drifted and bodies above i not yet; test particles
+ * see every massive body drifted.  nbody <= GAMD_MAX_MASSIVE.  w_out (6,
+ * nsave, n) and w_final (6, n) as gamd_leapfrog. */
+int gamd_nbody_leapfrog(const gamd_potential *ext, const gamd_potential *const *bodies, int nbody,
+                        const double *w0, size_t n, double t0, double dt, int64_t n_steps,
+                        int64_t save_stride, double *w_out, double *w_final, void *stream);
+
+/* Replaces direct_nbody_dop853 (dynamics/nbody/nbody.pyx:30-115): the n
+ * bodies form ONE 6n-dimensional system (body-major state, the reference's
+ * (n, 6) rows) with one shared adaptive step and Fwrapper_direct_nbody
+ * forces (dop853.cpp:990-1006), integrated from t0 to t1 with initial step
+ * h0 (nonzero; the reference passes t[1] - t[0]), uround = machine epsilon,
+ * nmax / nstiff / hmax as gamd_dop853 (direct_nbody_dop853 passes nstiff
+ * -1: off).  n <= GAMD_MAX_BODIES, nbody <= GAMD_MAX_MASSIVE.  Dense output
+ * at the n_out times t_out into w_out (6, n_out, n) when w_out is non-NULL;
+ * w_final (6, n); status (1) gala's idid; stats (4) nullable. */
+int gamd_nbody_dop853(const gamd_potential *ext, const gamd_potential *const *bodies, int nbody,
+                      const double *w0, size_t n, double t0, double t1, double h0, double rtol,
+                      double atol, int64_t nmax, int64_t nstiff, double hmax, const double *t_out,
+                      size_t n_out, double *w_out, double *w_final, int32_t *status,
+                      int64_t *stats, void *stream);
+
 /* ---- mock streams ------------------------------------------------------- */
 /* Replaces mockstream_leapfrog's release-and-integrate loop
- * (mockstream.pyx:551-599) for one massive body (the progenitor, whose
- * potential is prog, evaluated around the progenitor's position,
- * cpotential.cpp:418-442) and massless stream particles.
- *   prog_traj (6, nfull): progenitor states on the forward grid t0 + k dt
+ * (mockstream.pyx:551-599): the system bodies 0..nbody-1 (body 0 the
+ * progenitor, then any perturbers; potentials as above) and massless stream
+ * particles.
+ *   body_traj (6, nfull, nbody): the bodies on the forward grid t0 + k dt
  *     (the leapfrog_integrate_nbody trajectory of mockstream.pyx:536-549);
  *   w0 (6, n): particle states at release; rel_t (n): release times;
  *   out (6, n): particle states at tfinal.
- * Every particle integrates the progenitor again from its release state
- * alongside itself, exactly as each release batch does in the reference. */
-int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *prog,
-                             const double *prog_traj, size_t nfull, double t0, double dt,
-                             const double *w0, const double *rel_t, size_t n, double tfinal,
-                             double *out, void *stream);
+ * Every particle integrates the bodies again from their states at its
+ * release alongside itself (bodies first, in index order), exactly as each
+ * release batch does in the reference.  nbody <= GAMD_MAX_MASSIVE. */
+int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *const *bodies,
+                             int nbody, const double *body_traj, size_t nfull, double t0,
+                             double dt, const double *w0, const double *rel_t, size_t n,
+                             double tfinal, double *out, void *stream);
 
 /* Replaces the release loop of mockstream_dop853 (mockstream.pyx:286-303),
- * gala's default mock-stream integrator: release i's system is body 0, the
- * progenitor at prog_w[i] (its state at the release time from the dense
- * output of the progenitor's own DOP853 orbit, mockstream.pyx:274-282), plus
- * the nstream[i] particles in rows offset[i].. of stream_w0 (ntot, 6); the
- * (1 + k, 6) system is integrated with one shared adaptive step from
- * rel_t[i] to tfinal with the arguments dop853_step passes (dop853.pyx:29-79:
- * h = dt0, uround 2.3e-16, nstiff 1, hmax |tfinal - t|) and
- * Fwrapper_direct_nbody forces (dop853.cpp:990-1006; particles feel prog
- * centred on body 0).  All arrays are device arrays; nstream[i] <= 3
- * (status GAMD_ERR_UNSUPPORTED otherwise).  out (ntot, 6): particle states
- * at tfinal; prog_final (nrel, 6): body 0 of each system at tfinal;
- * status (nrel): gala's idid (0 for releases without particles); stats
- * (nrel, 4) nullable. */
-int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *prog,
-                           const double *prog_w, size_t nrel, const double *rel_t,
+ * gala's default mock-stream integrator: release i's system is the bodies
+ * 0..nbody-1 at body_w[i] (nbody, 6) (their states at the release time from
+ * the dense output of the bodies' own DOP853 integration,
+ * mockstream.pyx:274-282) followed by the nstream[i] particles in rows
+ * offset[i].. of stream_w0 (ntot, 6); the (nbody + k, 6) system is
+ * integrated with one shared adaptive step from rel_t[i] to tfinal with the
+ * arguments dop853_step passes (dop853.pyx:29-79: h = dt0, uround 2.3e-16,
+ * nstiff 1, hmax |tfinal - t|) and Fwrapper_direct_nbody forces
+ * (dop853.cpp:990-1006).  All arrays are device arrays; nbody +
+ * nstream[i] <= GAMD_MAX_BODIES (status GAMD_ERR_UNSUPPORTED otherwise),
+ * nbody <= GAMD_MAX_MASSIVE.  out (ntot, 6): particle states at tfinal;
+ * body_final (nrel, nbody, 6): the bodies of each system at tfinal; status
+ * (nrel): gala's idid (0 for releases without particles); stats (nrel, 4)
+ * nullable. */
+int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *const *bodies,
+                           int nbody, const double *body_w, size_t nrel, const double *rel_t,
                            const int32_t *nstream, const int64_t *offset, const double *stream_w0,
                            double tfinal, double dt0, double rtol, double atol, int64_t nmax,
-                           double *out, double *prog_final, int32_t *status, int64_t *stats,
+                           double *out, double *body_final, int32_t *status, int64_t *stats,
                            void *stream);
 
 /* ---- learned SCF-force surrogate --------------------------------------- */

@@ -17,6 +17,8 @@
  *   Ruth4          integrate/cyintegrators/ruth4.pyx:24-106
  *   DOP853         integrate/cyintegrators/dopri/dop853.cpp:18-650, 653-866,
  *                  869-904; dop853.pyx:90-250; hamiltonian/src/chamiltonian.cpp:38-57
+ *   N-body         cpotential.cpp:389-442, dop853.cpp:990-1006,
+ *                  dynamics/nbody/nbody.pyx:30-115
  *   mock stream    dynamics/mockstream/mockstream.pyx:176-303, 442-620,
  *                  df.pyx:61-106, 363-456, _coord.pyx:23-48
  *   SCF            potential/scf/src/bfe.cpp:20-259, bfe_helper.cpp:14-90
@@ -570,7 +572,8 @@ typedef struct orc_sys {
     void (*f)(const struct orc_sys *S, const double *y, double *dy);
     const orc_pot *P;    /* external potential */
     int nb;              /* orbits (Fwrapper_T) or bodies (Fwrapper_direct_nbody) */
-    const orc_pot *prog; /* n-body systems: the potential of body 0, or NULL */
+    const orc_pot *const *bodies; /* n-body systems: potentials of bodies 0..nbody-1 */
+    int nbody;                    /* (a NULL entry is a massless body) */
 } orc_sys;
 
 /* hamiltonian_gradient_T for the static frame (chamiltonian.cpp:38-57,
@@ -590,19 +593,19 @@ static void orc_fcn(const orc_pot *P, int nb, const double *y, double *f) {
 
 static void sys_orbits(const orc_sys *S, const double *y, double *f) { orc_fcn(S->P, S->nb, y, f); }
 
-static void particle_grad(const orc_pot *ext, const orc_pot *prog, const double *q,
-                          const double *xp, double *g);
+static void nbody_grad(const orc_pot *ext, const orc_pot *const *bodies, int nbody, int self,
+                       const double *q, const double *w, double *g);
 
 /* Fwrapper_direct_nbody (dop853.cpp:990-1006) for bodies in (nb, 6) rows:
  * f = (p, -grad Phi_ext(q)) per body, then c_nbody_acceleration
- * (cpotential.cpp:389-415) adds -grad of body 0's potential, centred on
- * body 0, to every other body (the bodies after 0 are massless). */
+ * (cpotential.cpp:389-415) subtracts, body by body in index order, the
+ * gradient of every other body's potential centred on that body.
+ * -(g_ext + f_0 + f_1 ...) equals (-g_ext) + (-f_0) + (-f_1) ... exactly. */
 static void sys_nbody(const orc_sys *S, const double *y, double *f) {
     for (int i = 0; i < S->nb; i++) {
         const double *q = y + 6 * i;
         double g[3];
-        if (i == 0 || !S->prog) grad1(S->P, q, g);
-        else particle_grad(S->P, S->prog, q, y, g);
+        nbody_grad(S->P, S->bodies, S->nbody, i, q, y, g);
         f[6 * i + 0] = q[3];
         f[6 * i + 1] = q[4];
         f[6 * i + 2] = q[5];
@@ -910,64 +913,121 @@ void orc_dop853_orbits(const orc_pot *P, const double *w0, size_t N, const doubl
 /* ------------------------------------------------------------------------ */
 /* Mock stream (mockstream.pyx, df.pyx, leapfrog.pyx N-body helpers)       */
 
-/* Gradient felt by a massless particle at q from the external potential
- * plus the progenitor potential centred on xp (c_gradient then
- * c_nbody_gradient_symplectic, leapfrog.pyx:151-152, cpotential.cpp:418-442).
- * The progenitor's own R is applied as in apply_shift_rotate. */
-static void particle_grad(const orc_pot *ext, const orc_pot *prog, const double *q,
-                          const double *xp, double *g) {
+/* Gradient at q felt by body `self` (-1 for none) of an N-body system whose
+ * bodies sit in rows of 6 in w: the external potential (c_gradient), then for
+ * every body j < nbody other than self with a potential, that potential with
+ * every component shifted to body j's position (do_shift_rotate = 1, q0 =
+ * body j, its own R kept), added in body order (c_nbody_gradient_symplectic,
+ * leapfrog.pyx:151-152, cpotential.cpp:418-442). */
+static void nbody_grad(const orc_pot *ext, const orc_pot *const *bodies, int nbody, int self,
+                       const double *q, const double *w, double *g) {
     grad1(ext, q, g);
-    if (!prog || prog->n == 0) return;
-    orc_pot shifted = *prog;
-    for (int i = 0; i < shifted.n; i++) {
-        shifted.c[i].shift_rotate = 1;
-        shifted.c[i].q0[0] = xp[0];
-        shifted.c[i].q0[1] = xp[1];
-        shifted.c[i].q0[2] = xp[2];
+    for (int j = 0; j < nbody; j++) {
+        const orc_pot *bp = bodies ? bodies[j] : NULL;
+        if (j == self || !bp || bp->n == 0) continue;
+        orc_pot shifted = *bp;
+        for (int i = 0; i < shifted.n; i++) {
+            shifted.c[i].shift_rotate = 1;
+            shifted.c[i].q0[0] = w[6 * j + 0];
+            shifted.c[i].q0[1] = w[6 * j + 1];
+            shifted.c[i].q0[2] = w[6 * j + 2];
+        }
+        double f2[3];
+        grad1(&shifted, q, f2);
+        g[0] += f2[0];
+        g[1] += f2[1];
+        g[2] += f2[2];
     }
-    double f2[3];
-    grad1(&shifted, q, f2);
-    g[0] += f2[0];
-    g[1] += f2[1];
-    g[2] += f2[2];
 }
 
-/* mockstream_leapfrog (mockstream.pyx:442-620) for one massive body (the
- * progenitor, body 0) and massless stream particles.
- *   full_time (nfull): the forward time grid; prog_w0[6]: progenitor state at
- *   full_time[0]; spawn_t/nstream (nrel): release times and counts;
- *   stream_w0 (ntot,6) AoS; out: prog_final[6], stream_final (ntot,6).
- * Each release batch integrates the progenitor again from its state on the
- * full-grid trajectory together with the batch's particles (:555-599). */
-void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const double *full_time,
-                             size_t nfull, const double *prog_w0, size_t nrel,
-                             const double *spawn_t, const int *nstream, const double *stream_w0,
-                             double tfinal, double *prog_final, double *stream_final) {
-    double dt = full_time[1] - full_time[0];
-    /* leapfrog_integrate_nbody of the progenitor alone (:536-540) */
-    double *traj = (double *)malloc(sizeof(double) * 6 * nfull);
-    {
-        double x[3], v[3], vh[3], g[3];
-        for (int k = 0; k < 3; k++) {
-            x[k] = prog_w0[k];
-            v[k] = prog_w0[3 + k];
-        }
-        for (int k = 0; k < 6; k++) traj[k] = prog_w0[k];
-        grad1(ext, x, g);
-        for (int k = 0; k < 3; k++) vh[k] = v[k] - g[k] * dt / 2.0;
-        for (size_t j = 1; j < nfull; j++) {
-            for (int k = 0; k < 3; k++) x[k] = x[k] + vh[k] * dt;
-            grad1(ext, x, g);
+/* leapfrog_integrate_nbody (leapfrog.pyx:161-257): n bodies in (n, 6) rows,
+ * bodies 0..nbody-1 with the potentials `bodies` (NULL = massless), the rest
+ * massless.  Half kicks for every body (c_init_velocity_nbody, :126-137), then
+ * per step every body in index order drifts, takes its gradient against the
+ * others' current positions and kicks (c_leapfrog_step_nbody, :140-158).
+ * out (ntimes, n, 6) when save_all (row 0 = w0), else (n, 6). */
+void orc_nbody_leapfrog(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                        const double *w0, int n, const double *t, size_t ntimes, int save_all,
+                        double *out) {
+    const double dt = t[1] - t[0]; /* :217 */
+    double *w = (double *)malloc(sizeof(double) * 6 * (n > 0 ? n : 1));
+    double *vh = (double *)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    memcpy(w, w0, sizeof(double) * 6 * n);
+    if (save_all) memcpy(out, w0, sizeof(double) * 6 * n);
+    for (int i = 0; i < n; i++) {
+        double g[3];
+        nbody_grad(ext, bodies, nbody, i, &w[6 * i], w, g);
+        for (int k = 0; k < 3; k++) vh[3 * i + k] = w[6 * i + 3 + k] - g[k] * dt / 2.0;
+    }
+    for (size_t j = 1; j < ntimes; j++) {
+        for (int i = 0; i < n; i++) {
+            double g[3];
+            for (int k = 0; k < 3; k++) w[6 * i + k] = w[6 * i + k] + vh[3 * i + k] * dt;
+            nbody_grad(ext, bodies, nbody, i, &w[6 * i], w, g);
             for (int k = 0; k < 3; k++) {
-                v[k] = vh[k] - g[k] * dt / 2.0;
-                vh[k] = vh[k] - g[k] * dt;
+                w[6 * i + 3 + k] = vh[3 * i + k] - g[k] * dt / 2.0;
+                vh[3 * i + k] = vh[3 * i + k] - g[k] * dt;
             }
-            for (int k = 0; k < 3; k++) {
-                traj[6 * j + k] = x[k];
-                traj[6 * j + 3 + k] = v[k];
-            }
+            if (save_all)
+                for (int k = 0; k < 6; k++) out[(j * n + i) * 6 + k] = w[6 * i + k];
         }
     }
+    if (!save_all) memcpy(out, w, sizeof(double) * 6 * n);
+    free(w);
+    free(vh);
+}
+
+/* direct_nbody_dop853 (nbody.pyx:30-115) through dop853_helper
+ * (dop853.pyx:90-193): the n bodies (n, 6) are one system with
+ * Fwrapper_direct_nbody forces, integrated over t with h = t[1] - t[0],
+ * uround = eps, hmax = dt_max (0: the whole span) and the stiffness test off
+ * (nstiff = -1); dense output at t.  out (ntimes, n, 6) when save_all, else
+ * (n, 6).  Returns the idid; stats[4] nullable. */
+int orc_nbody_dop853(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                     const double *w0, int n, const double *t, size_t ntimes, double atol,
+                     double rtol, long nmax, double dt_max, int save_all, double *out,
+                     long *stats) {
+    if (nmax == 0) nmax = 1000000;                               /* :673-674 */
+    const long nstiff = nmax + 10;                               /* -1: :693-694 */
+    double hmax = dt_max == 0.0 ? t[ntimes - 1] - t[0] : dt_max; /* :787-788 */
+    const int m = 6 * n;
+    double *y = (double *)malloc(sizeof(double) * m);
+    double *work = (double *)malloc(sizeof(double) * m * 19);
+    memcpy(y, w0, sizeof(double) * m);
+    orc_sys S = {m, sys_nbody, ext, n, bodies, nbody};
+    orc_dopstats st;
+    int idid = orc_dopcor(&S, t[0], y, t[ntimes - 1], hmax, t[1] - t[0], rtol, atol, nmax,
+                          nstiff, 2.220446049250313e-16, save_all ? t : NULL, (int)ntimes,
+                          save_all ? out : NULL, work, &st);
+    if (!save_all) memcpy(out, y, sizeof(double) * m);
+    if (stats) {
+        stats[0] = st.nstep;
+        stats[1] = st.naccpt;
+        stats[2] = st.nrejct;
+        stats[3] = st.nfcn;
+    }
+    free(y);
+    free(work);
+    return idid;
+}
+
+/* mockstream_leapfrog (mockstream.pyx:442-620): nbody system bodies (body 0
+ * the progenitor, then perturbers; potentials `bodies`, NULL = massless)
+ * and massless stream particles.
+ *   full_time (nfull): the forward time grid; body_w0 (nbody, 6): the bodies
+ *   at full_time[0]; spawn_t/nstream (nrel): release times and counts;
+ *   stream_w0 (ntot,6) AoS; out: body_final (nbody, 6), stream_final (ntot,6).
+ * The bodies' trajectory comes from leapfrog_integrate_nbody over full_time
+ * (:536-549); each release batch integrates the bodies again from their
+ * states on that trajectory together with the batch's particles (:555-599). */
+void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                             const double *full_time, size_t nfull, const double *body_w0,
+                             size_t nrel, const double *spawn_t, const int *nstream,
+                             const double *stream_w0, double tfinal, double *body_final,
+                             double *stream_final) {
+    double dt = full_time[1] - full_time[0];
+    double *traj = (double *)malloc(sizeof(double) * 6 * nbody * nfull); /* (nfull, nbody, 6) */
+    orc_nbody_leapfrog(ext, bodies, nbody, body_w0, nbody, full_time, nfull, 1, traj);
     size_t *offs = (size_t *)malloc(sizeof(size_t) * (nrel + 1));
     offs[0] = 0;
     for (size_t i = 0; i < nrel; i++) offs[i + 1] = offs[i] + (size_t)nstream[i];
@@ -981,17 +1041,17 @@ void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const doub
         int ns = nstream[i];
         if (ns == 0) continue;
         int idx = (int)((spawn_t[i] - full_time[0]) / dt + 0.5); /* :546 */
-        int nbod = 1 + ns;
+        int nbod = nbody + ns;
         double *w = (double *)malloc(sizeof(double) * 6 * nbod);
         double *vh = (double *)malloc(sizeof(double) * 3 * nbod);
-        for (int k = 0; k < 6; k++) w[k] = traj[6 * idx + k];
+        for (int k = 0; k < 6 * nbody; k++) w[k] = traj[(size_t)idx * 6 * nbody + k];
         for (int j = 0; j < ns; j++)
-            for (int k = 0; k < 6; k++) w[6 * (1 + j) + k] = stream_w0[6 * (offs[i] + j) + k];
+            for (int k = 0; k < 6; k++)
+                w[6 * (nbody + j) + k] = stream_w0[6 * (offs[i] + j) + k];
         /* c_init_velocity_nbody for each body (:571-578) */
         for (int j = 0; j < nbod; j++) {
             double g[3];
-            if (j == 0) grad1(ext, &w[0], g);
-            else particle_grad(ext, prog, &w[6 * j], &w[0], g);
+            nbody_grad(ext, bodies, nbody, j, &w[6 * j], w, g);
             for (int k = 0; k < 3; k++) vh[3 * j + k] = w[6 * j + 3 + k] - g[k] * dt / 2.0;
         }
         int n_steps = (int)((tfinal - spawn_t[i]) / dt + 0.5); /* :581 */
@@ -999,8 +1059,7 @@ void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const doub
             for (int j = 0; j < nbod; j++) { /* c_leapfrog_step_nbody, :140-158 */
                 double g[3];
                 for (int k = 0; k < 3; k++) w[6 * j + k] = w[6 * j + k] + vh[3 * j + k] * dt;
-                if (j == 0) grad1(ext, &w[0], g);
-                else particle_grad(ext, prog, &w[6 * j], &w[0], g);
+                nbody_grad(ext, bodies, nbody, j, &w[6 * j], w, g);
                 for (int k = 0; k < 3; k++) {
                     w[6 * j + 3 + k] = vh[3 * j + k] - g[k] * dt / 2.0;
                     vh[3 * j + k] = vh[3 * j + k] - g[k] * dt;
@@ -1008,9 +1067,10 @@ void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const doub
             }
         }
         for (int j = 0; j < ns; j++)
-            for (int k = 0; k < 6; k++) stream_final[6 * (offs[i] + j) + k] = w[6 * (1 + j) + k];
+            for (int k = 0; k < 6; k++)
+                stream_final[6 * (offs[i] + j) + k] = w[6 * (nbody + j) + k];
         if ((long)i == last_rel)
-            for (int k = 0; k < 6; k++) prog_final[k] = w[k];
+            for (int k = 0; k < 6 * nbody; k++) body_final[k] = w[k];
         free(w);
         free(vh);
     }
@@ -1019,20 +1079,20 @@ void orc_mockstream_leapfrog(const orc_pot *ext, const orc_pot *prog, const doub
 }
 
 /* mockstream_dop853's release loop (mockstream.pyx:286-303): for each
- * release i with nstream[i] > 0, body 0 (the progenitor) starts from
- * prog_w[i] (6 values, its dense-output state at the release time,
+ * release i with nstream[i] > 0, the nbody bodies start from body_w[i]
+ * (nbody, 6): their dense-output states at the release time,
  * mockstream.pyx:274-282) and the release's particles (the next nstream[i]
- * rows of stream_w0, (ntot, 6)) follow; the (1 + k, 6) system is integrated
- * by dop853_step (dop853.pyx:29-79) from stream_t1[i] to tfinal with h = dt0
- * and the defaults dop853() picks for it (uround 2.3e-16, hmax tfinal - t1,
- * dop853.cpp:733-792) and nstiff = 1 as dop853_step passes.
- * out (ntot, 6); prog_final[6]: body 0 after the last release's system;
- * status[nrel] (0 where nstream[i] = 0), stats (nrel, 4) nullable. */
-void orc_mockstream_dop853(const orc_pot *ext, const orc_pot *prog, const double *prog_w,
-                           size_t nrel, const double *stream_t1, const int *nstream,
-                           const double *stream_w0, double tfinal, double dt0, double atol,
-                           double rtol, long nmax, double *out, double *prog_final, int *status,
-                           long *stats) {
+ * rows of stream_w0, (ntot, 6)) follow; the (nbody + k, 6) system is
+ * integrated by dop853_step (dop853.pyx:29-79) from stream_t1[i] to tfinal
+ * with h = dt0 and the defaults dop853() picks for it (uround 2.3e-16, hmax
+ * tfinal - t1, dop853.cpp:733-792) and nstiff = 1 as dop853_step passes.
+ * out (ntot, 6); body_final (nbody, 6): the bodies after the last release's
+ * system; status[nrel] (0 where nstream[i] = 0), stats (nrel, 4) nullable. */
+void orc_mockstream_dop853(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                           const double *body_w, size_t nrel, const double *stream_t1,
+                           const int *nstream, const double *stream_w0, double tfinal,
+                           double dt0, double atol, double rtol, long nmax, double *out,
+                           double *body_final, int *status, long *stats) {
     if (nmax == 0) nmax = 1000000; /* :673-674 */
     size_t *offs = (size_t *)malloc(sizeof(size_t) * (nrel + 1));
     offs[0] = 0;
@@ -1046,20 +1106,21 @@ void orc_mockstream_dop853(const orc_pot *ext, const orc_pot *prog, const double
         int k = nstream[i];
         if (status) status[i] = 0;
         if (k <= 0) continue;
-        int nb = 1 + k, n = 6 * nb;
+        int nb = nbody + k, n = 6 * nb;
         double *y = (double *)malloc(sizeof(double) * n);
         double *work = (double *)malloc(sizeof(double) * n * 19);
-        for (int c = 0; c < 6; c++) y[c] = prog_w[6 * i + c];
+        for (int c = 0; c < 6 * nbody; c++) y[c] = body_w[6 * nbody * i + c];
         for (int j = 0; j < k; j++)
-            for (int c = 0; c < 6; c++) y[6 * (1 + j) + c] = stream_w0[6 * (offs[i] + j) + c];
-        orc_sys S = {n, sys_nbody, ext, nb, prog};
+            for (int c = 0; c < 6; c++)
+                y[6 * (nbody + j) + c] = stream_w0[6 * (offs[i] + j) + c];
+        orc_sys S = {n, sys_nbody, ext, nb, bodies, nbody};
         orc_dopstats st;
         int idid = orc_dopcor(&S, stream_t1[i], y, tfinal, tfinal - stream_t1[i], dt0, rtol, atol,
                               nmax, 1, 2.3e-16, NULL, 0, NULL, work, &st);
         for (int j = 0; j < k; j++)
-            for (int c = 0; c < 6; c++) out[6 * (offs[i] + j) + c] = y[6 * (1 + j) + c];
+            for (int c = 0; c < 6; c++) out[6 * (offs[i] + j) + c] = y[6 * (nbody + j) + c];
         if ((long)i == last_rel)
-            for (int c = 0; c < 6; c++) prog_final[c] = y[c];
+            for (int c = 0; c < 6 * nbody; c++) body_final[c] = y[c];
         if (status) status[i] = idid;
         if (stats) {
             stats[4 * i + 0] = st.nstep;

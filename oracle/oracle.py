@@ -66,11 +66,17 @@ def lib(fast=False):
                                  C.c_long, C.c_long, C.c_double, C.c_int, C.c_int, dp, ip, lp]
         L.orc_dop853_orbits.argtypes = [P, dp, C.c_size_t, dp, dp, dp, C.c_double, C.c_double,
                                         C.c_long, C.c_long, dp, ip, lp]
-        L.orc_mockstream_leapfrog.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, C.c_size_t, dp,
-                                              ip, dp, C.c_double, dp, dp]
-        L.orc_mockstream_dop853.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, ip, dp,
+        L.orc_mockstream_leapfrog.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_size_t, dp,
+                                              C.c_size_t, dp, ip, dp, C.c_double, dp, dp]
+        L.orc_mockstream_dop853.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_size_t, dp, ip, dp,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_long, dp, dp, ip, lp]
+        L.orc_nbody_leapfrog.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                         C.c_int, dp]
+        L.orc_nbody_dop853.restype = C.c_int
+        L.orc_nbody_dop853.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                       C.c_double, C.c_double, C.c_long, C.c_double, C.c_int, dp,
+                                       lp]
         L.orc_fardal_sample.restype = C.c_long
         L.orc_fardal_sample.argtypes = [P, C.c_double, dp, dp, dp, dp, ip, C.c_size_t, C.c_int,
                                         C.c_int, C.c_int, dp, dp, dp, dp]
@@ -100,9 +106,14 @@ def ref_lib():
         L.ref_hessian.argtypes = [P, C.c_double, dp, dp]
         L.ref_d2_dr2.restype = C.c_double
         L.ref_d2_dr2.argtypes = [P, C.c_double, dp]
-        L.ref_mockstream_dop853.argtypes = [P, C.c_void_p, dp, C.c_size_t, dp, ip, dp,
+        L.ref_mockstream_dop853.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_size_t, dp, ip, dp,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_long, dp, dp, ip]
+        L.ref_nbody_dop853.restype = C.c_int
+        L.ref_nbody_dop853.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                       C.c_double, C.c_double, C.c_long, C.c_double, C.c_int, dp]
+        L.ref_nbody_leapfrog.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                         C.c_int, dp]
         L.ref_dop853.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_double, C.c_double,
                                  C.c_long, C.c_long, C.c_double, C.c_int, C.c_int, dp, ip]
         _LIBS["ref"] = L
@@ -295,50 +306,104 @@ def dop853_orbits(components, w0, t0, t1, h0, atol, rtol, nmax=0, nstiff=0, fast
     return out, status, stats
 
 
-def mockstream_leapfrog(ext_components, prog_components, full_time, prog_w0, spawn_t, nstream,
-                        stream_w0, tfinal, fast=False):
-    """Returns (prog_final (6,), stream_final (ntot,6))."""
+def _bodies(bodies):
+    """Body potentials for the N-body functions: a list with one component
+    list per body (None or [] = massless).  A bare component list (or None)
+    is the progenitor alone.  Returns (ctypes array of orc_pot*, count,
+    keepalive)."""
+    if bodies is None or (len(bodies) > 0 and isinstance(bodies[0], dict)):
+        bodies = [bodies]
+    pots = [make_pot(b) if b else None for b in bodies]
+    arr = (C.c_void_p * max(len(pots), 1))()
+    for j, p in enumerate(pots):
+        arr[j] = C.addressof(p) if p is not None else None
+    return arr, len(pots), pots
+
+
+def nbody_leapfrog(ext_components, bodies, w0, t, save_all=True, ref=False):
+    """leapfrog_integrate_nbody (leapfrog.pyx:161-257).  w0 (n, 6) rows,
+    bodies: potentials of bodies 0..K-1 (see _bodies).  Returns (ntimes, n, 6)
+    when save_all, else (n, 6)."""
     P = make_pot(ext_components)
-    PP = make_pot(prog_components) if prog_components else None
+    arr, K, keep = _bodies(bodies)
+    w0 = _c(w0).reshape(-1, 6)
+    n = w0.shape[0]
+    t = _c(t)
+    out = np.zeros((len(t), n, 6) if save_all else (n, 6))
+    L = ref_lib() if ref else lib()
+    fn = L.ref_nbody_leapfrog if ref else L.orc_nbody_leapfrog
+    fn(C.byref(P), arr, K, w0, n, t, len(t), int(save_all), out.reshape(-1))
+    return out
+
+
+def nbody_dop853(ext_components, bodies, w0, t, atol=1e-10, rtol=1e-10, nmax=0, dt_max=0.0,
+                 save_all=True, ref=False):
+    """direct_nbody_dop853 (nbody.pyx:30-115).  w0 (n, 6).  Returns (out
+    ((ntimes, n, 6) or (n, 6)), idid, stats (4,) or None with ref)."""
+    P = make_pot(ext_components)
+    arr, K, keep = _bodies(bodies)
+    w0 = _c(w0).reshape(-1, 6)
+    n = w0.shape[0]
+    t = _c(t)
+    out = np.zeros((len(t), n, 6) if save_all else (n, 6))
+    if ref:
+        idid = ref_lib().ref_nbody_dop853(C.byref(P), arr, K, w0, n, t, len(t), atol, rtol, nmax,
+                                          dt_max, int(save_all), out.reshape(-1))
+        return out, idid, None
+    stats = np.zeros(4, dtype=np.int64)
+    idid = lib().orc_nbody_dop853(C.byref(P), arr, K, w0, n, t, len(t), atol, rtol, nmax,
+                                  dt_max, int(save_all), out.reshape(-1), stats)
+    return out, idid, stats
+
+
+def mockstream_leapfrog(ext_components, bodies, full_time, body_w0, spawn_t, nstream,
+                        stream_w0, tfinal, fast=False):
+    """mockstream_leapfrog (mockstream.pyx:442-620).  bodies: the progenitor's
+    components (or None), or a list of per-body component lists (progenitor
+    first, see _bodies); body_w0 (K, 6) at full_time[0].  Returns (body_final
+    ((6,) for one body, else (K, 6)), stream_final (ntot, 6))."""
+    P = make_pot(ext_components)
+    arr, K, keep = _bodies(bodies)
     full_time = _c(full_time)
     spawn_t = _c(spawn_t)
     nstream = np.ascontiguousarray(nstream, dtype=np.int32)
     stream_w0 = _c(stream_w0).reshape(-1, 6)
-    prog_final = np.zeros(6)
+    body_final = np.zeros((K, 6))
     stream_final = np.zeros_like(stream_w0)
-    lib(fast).orc_mockstream_leapfrog(C.byref(P), C.cast(C.byref(PP), C.c_void_p) if PP else None,
-                                      full_time, len(full_time), _c(prog_w0), len(spawn_t),
-                                      spawn_t, nstream, stream_w0, float(tfinal), prog_final,
+    lib(fast).orc_mockstream_leapfrog(C.byref(P), arr, K, full_time, len(full_time),
+                                      _c(body_w0).reshape(-1), len(spawn_t), spawn_t, nstream,
+                                      stream_w0, float(tfinal), body_final.reshape(-1),
                                       stream_final.reshape(-1))
-    return prog_final, stream_final
+    return (body_final[0] if K == 1 else body_final), stream_final
 
 
-def mockstream_dop853(ext_components, prog_components, prog_w, stream_t1, nstream, stream_w0,
+def mockstream_dop853(ext_components, bodies, body_w, stream_t1, nstream, stream_w0,
                       tfinal, dt0, atol=1e-10, rtol=1e-10, nmax=0, ref=False):
     """Release systems of mockstream_dop853 (mockstream.pyx:286-303).
-    prog_w (nrel, 6) progenitor states at the release times; stream_t1 (nrel,);
+    bodies as in mockstream_leapfrog; body_w (nrel, K, 6) (or (nrel, 6) for
+    one body) the bodies' states at the release times; stream_t1 (nrel,);
     nstream (nrel,) int; stream_w0 (ntot, 6).  Returns (out (ntot, 6),
-    prog_final (6,), status (nrel,), stats (nrel, 4) or None with ref)."""
+    body_final ((6,) for one body, else (K, 6)), status (nrel,), stats
+    (nrel, 4) or None with ref)."""
     P = make_pot(ext_components)
-    PP = make_pot(prog_components) if prog_components else None
-    prog_w = _c(prog_w).reshape(-1, 6)
-    nrel = prog_w.shape[0]
+    arr, K, keep = _bodies(bodies)
+    body_w = _c(body_w).reshape(-1, K * 6)
+    nrel = body_w.shape[0]
     t1 = _c(stream_t1)
     ns = np.ascontiguousarray(nstream, dtype=np.int32)
     sw = _c(stream_w0).reshape(-1, 6)
     out = np.zeros_like(sw)
-    pf = np.zeros(6)
+    bf = np.zeros((K, 6))
     status = np.zeros(nrel, dtype=np.int32)
+    one = (lambda x: x[0]) if K == 1 else (lambda x: x)
     if ref:
-        ref_lib().ref_mockstream_dop853(C.byref(P), C.cast(C.byref(PP), C.c_void_p) if PP else None,
-                                        prog_w, nrel, t1, ns, sw, tfinal, dt0, atol, rtol, nmax,
-                                        out, pf, status)
-        return out, pf, status, None
+        ref_lib().ref_mockstream_dop853(C.byref(P), arr, K, body_w, nrel, t1, ns, sw, tfinal, dt0,
+                                        atol, rtol, nmax, out, bf.reshape(-1), status)
+        return out, one(bf), status, None
     stats = np.zeros((nrel, 4), dtype=np.int64)
-    lib().orc_mockstream_dop853(C.byref(P), C.cast(C.byref(PP), C.c_void_p) if PP else None,
-                                prog_w, nrel, t1, ns, sw, tfinal, dt0, atol, rtol, nmax, out, pf,
-                                status, stats)
-    return out, pf, status, stats
+    lib().orc_mockstream_dop853(C.byref(P), arr, K, body_w, nrel, t1, ns, sw, tfinal, dt0, atol,
+                                rtol, nmax, out, bf.reshape(-1), status, stats)
+    return out, one(bf), status, stats
 
 
 def fardal_sample(components, G, prog_x, prog_v, prog_t, prog_m, nparticles, z, lead=True,

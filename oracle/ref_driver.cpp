@@ -203,58 +203,144 @@ void ref_dop853(const orc_pot *P, const double *w0, size_t N, const double *t, s
     destroy(rp);
 }
 
-// mockstream_dop853's release loop (mockstream.pyx:286-303): one call of
-// gala's dop853() per release through the arguments dop853_step passes
-// (dop853.pyx:29-79): Fwrapper_direct_nbody, norbits = 1 + k, nbody = 1,
-// uround/safe/fac/beta/hmax 0 (defaults), h = dt0, meth 0, nstiff 1.
-// prog_w (nrel, 6); stream_w0/out (ntot, 6); status[nrel].
-void ref_mockstream_dop853(const orc_pot *ext, const orc_pot *prog, const double *prog_w,
-                           size_t nrel, const double *stream_t1, const int *nstream,
-                           const double *stream_w0, double tfinal, double dt0, double atol,
-                           double rtol, long nmax, double *out, double *prog_final,
-                           int *status) {
-    RefPot *re = build(ext);
+// The body potentials of an N-body system as the reference holds them: one
+// CPotential per body, a null CPotential (null = 1, as NullWrapper) for a
+// massless body (mockstream.pyx _setup_particle_potentials; DirectNBody's
+// particle_potentials).  Bodies from nbody to n are massless.
+struct RefBodies {
+    std::vector<RefPot *> built;
+    CPotential *nullp;
+    std::vector<CPotential *> pots;
+    RefBodies(const orc_pot *const *bodies, int nbody, int n) {
+        nullp = allocate_cpotential(0);
+        nullp->null = 1;
+        nullp->n_dim = 3;
+        pots.assign((size_t)(n > nbody ? n : nbody), nullp);
+        for (int j = 0; j < nbody; j++) {
+            if (bodies && bodies[j]) {
+                RefPot *rp = build(bodies[j]);
+                built.push_back(rp);
+                pots[j] = rp->cp;
+            }
+        }
+    }
+    ~RefBodies() {
+        for (RefPot *rp : built) destroy(rp);
+        free_cpotential(nullp);
+    }
+};
+
+CFrameType static_frame() {
     CFrameType cf;
     cf.energy = (energyfunc)static_frame_hamiltonian;
     cf.gradient = (gradientfunc)static_frame_gradient;
     cf.hessian = NULL;
     cf.n_params = 0;
     cf.parameters = NULL;
+    return cf;
+}
+
+// mockstream_dop853's release loop (mockstream.pyx:286-303): one call of
+// gala's dop853() per release through the arguments dop853_step passes
+// (dop853.pyx:29-79): Fwrapper_direct_nbody, norbits = nbody + k, nbody,
+// uround/safe/fac/beta/hmax 0 (defaults), h = dt0, meth 0, nstiff 1.
+// body_w (nrel, nbody, 6); stream_w0/out (ntot, 6); body_final (nbody, 6);
+// status[nrel].
+void ref_mockstream_dop853(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                           const double *body_w, size_t nrel, const double *stream_t1,
+                           const int *nstream, const double *stream_w0, double tfinal,
+                           double dt0, double atol, double rtol, long nmax, double *out,
+                           double *body_final, int *status) {
+    RefPot *re = build(ext);
+    CFrameType cf = static_frame();
     size_t off = 0;
     for (size_t i = 0; i < nrel; i++) {
         const int k = nstream[i];
         status[i] = 0;
         if (k <= 0) continue;
-        const unsigned nb = 1 + (unsigned)k, n = 6 * nb;
-        // a fresh progenitor CPotential per release: c_nbody_acceleration
-        // re-points its origin at body 0 on every call
-        RefPot *rp = prog ? build(prog) : nullptr;
-        CPotential *nullp = allocate_cpotential(0);
-        nullp->null = 1;
-        nullp->n_dim = 3;
-        std::vector<CPotential *> pots(nb, nullp);
-        if (rp) {
-            pots[0] = rp->cp;
-        } else {
-            pots[0] = nullp;
-        }
+        const unsigned nb = (unsigned)(nbody + k), n = 6 * nb;
+        // fresh body CPotentials per release: c_nbody_acceleration re-points
+        // their origins at the bodies on every call
+        RefBodies rb(bodies, nbody, (int)nb);
         std::vector<double> w(n);
-        for (int c = 0; c < 6; c++) w[c] = prog_w[6 * i + c];
+        for (int c = 0; c < 6 * nbody; c++) w[c] = body_w[6 * nbody * i + c];
         for (int j = 0; j < k; j++)
-            for (int c = 0; c < 6; c++) w[6 * (1 + j) + c] = stream_w0[6 * (off + j) + c];
+            for (int c = 0; c < 6; c++) w[6 * (nbody + j) + c] = stream_w0[6 * (off + j) + c];
         double rt = rtol, at = atol;
-        int res = dop853(n, (FcnEqDiff)Fwrapper_direct_nbody, re->cp, &cf, nb, 1,
-                         (void *)pots.data(), stream_t1[i], w.data(), tfinal, &rt, &at, 0, NULL,
-                         0, NULL, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dt0, nmax, 0, 1, 0, NULL, 0, NULL,
-                         NULL, 0, NULL);
+        int res = dop853(n, (FcnEqDiff)Fwrapper_direct_nbody, re->cp, &cf, nb, (unsigned)nbody,
+                         (void *)rb.pots.data(), stream_t1[i], w.data(), tfinal, &rt, &at, 0,
+                         NULL, 0, NULL, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dt0, nmax, 0, 1, 0, NULL, 0,
+                         NULL, NULL, 0, NULL);
         status[i] = res;
         for (int j = 0; j < k; j++)
-            for (int c = 0; c < 6; c++) out[6 * (off + j) + c] = w[6 * (1 + j) + c];
-        for (int c = 0; c < 6; c++) prog_final[c] = w[c];
+            for (int c = 0; c < 6; c++) out[6 * (off + j) + c] = w[6 * (nbody + j) + c];
+        for (int c = 0; c < 6 * nbody; c++) body_final[c] = w[c];
         off += (size_t)k;
-        free_cpotential(nullp);
-        if (rp) destroy(rp);
     }
+    destroy(re);
+}
+
+// direct_nbody_dop853 (nbody.pyx:30-115) through dop853_helper's dop853()
+// call (dop853.pyx:157-182): the n bodies (n, 6) as one system with
+// Fwrapper_direct_nbody, h = t[1] - t[0], uround eps, hmax dt_max, nstiff -1,
+// dense output at t.  out (ntimes, n, 6) when save_all, else (n, 6).
+// Returns the idid.
+int ref_nbody_dop853(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                     const double *w0, int n, const double *t, size_t ntimes, double atol,
+                     double rtol, long nmax, double dt_max, int save_all, double *out) {
+    RefPot *re = build(ext);
+    CFrameType cf = static_frame();
+    RefBodies rb(bodies, nbody, n);
+    const unsigned size = 6 * (unsigned)n;
+    std::vector<double> w(w0, w0 + size);
+    Dop853DenseState *state = save_all ? dop853_dense_state_alloc(size, size) : NULL;
+    int res = dop853(size, (FcnEqDiff)Fwrapper_direct_nbody, re->cp, &cf, (unsigned)n,
+                     (unsigned)nbody, (void *)rb.pots.data(), t[0], w.data(), t[ntimes - 1],
+                     &rtol, &atol, 0, NULL, 0, NULL, 2.220446049250313e-16, 0.0, 0.0, 0.0, 0.0,
+                     dt_max, t[1] - t[0], nmax, 1, -1, save_all ? size : 0, NULL, 0, state,
+                     const_cast<double *>(t), (int)ntimes, save_all ? out : NULL);
+    if (state) dop853_dense_state_free(state, size);
+    if (!save_all) memcpy(out, w.data(), sizeof(double) * size);
+    destroy(re);
+    return res;
+}
+
+// leapfrog_integrate_nbody (leapfrog.pyx:161-257): the Cython loops
+// (c_init_velocity_nbody :126-137, c_leapfrog_step_nbody :140-158) restated
+// around gala's compiled c_gradient and c_nbody_gradient_symplectic
+// (cpotential.cpp:418-442).  out (ntimes, n, 6) when save_all, else (n, 6).
+void ref_nbody_leapfrog(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                        const double *w0, int n, const double *t, size_t ntimes, int save_all,
+                        double *out) {
+    RefPot *re = build(ext);
+    RefBodies rb(bodies, nbody, n);
+    const double dt = t[1] - t[0];
+    std::vector<double> w(w0, w0 + 6 * n), vh(3 * (size_t)n);
+    if (save_all) memcpy(out, w0, sizeof(double) * 6 * n);
+    double grad[3];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) grad[k] = 0.0;
+        c_gradient(re->cp, 1, t[0], &w[6 * i], grad);
+        c_nbody_gradient_symplectic(rb.pots.data(), t[0], &w[6 * i], w.data(), nbody, i, 3,
+                                    grad);
+        for (int k = 0; k < 3; k++) vh[3 * i + k] = w[6 * i + 3 + k] - grad[k] * dt / 2.;
+    }
+    for (size_t j = 1; j < ntimes; j++) {
+        for (int i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++) grad[k] = 0.0;
+            for (int k = 0; k < 3; k++) w[6 * i + k] = w[6 * i + k] + vh[3 * i + k] * dt;
+            c_gradient(re->cp, 1, t[j], &w[6 * i], grad);
+            c_nbody_gradient_symplectic(rb.pots.data(), t[j], &w[6 * i], w.data(), nbody, i, 3,
+                                        grad);
+            for (int k = 0; k < 3; k++) {
+                w[6 * i + 3 + k] = vh[3 * i + k] - grad[k] * dt / 2.;
+                vh[3 * i + k] = vh[3 * i + k] - grad[k] * dt;
+            }
+            if (save_all)
+                for (int k = 0; k < 6; k++) out[(j * n + i) * 6 + k] = w[6 * i + k];
+        }
+    }
+    if (!save_all) memcpy(out, w.data(), sizeof(double) * 6 * n);
     destroy(re);
 }
 

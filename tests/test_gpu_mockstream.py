@@ -75,7 +75,8 @@ def test_release_and_integrate_kernel_matches_oracle():
     tr = dev.to_dev(np.ascontiguousarray(traj_h))
     w0d = dev.to_dev(w0)
     rtd = dev.to_dev(rel_t)
-    _lib.check(lib.gamd_mockstream_leapfrog(ext.c_handle, prog_pot.c_handle, tr.data_ptr(), nfull,
+    _lib.check(lib.gamd_mockstream_leapfrog(ext.c_handle, _lib.body_array([prog_pot.c_handle]),
+                                            1, tr.data_ptr(), nfull,
                                             full_t[0], 1.0, w0d.data_ptr(), rtd.data_ptr(), n,
                                             full_t[-1], out.data_ptr(), dev.stream_ptr()),
                "gamd_mockstream_leapfrog")
@@ -125,10 +126,16 @@ def test_full_size_stream_against_oracle():
     gen, prog_w0, prog_mass, b, dt, nsteps = _generator(11)
     stream, prog = gen.run(prog_w0, prog_mass, dt=dt, n_steps=nsteps, Integrator="leapfrog")
     assert stream.xyz.shape == (3, 8002)
-    # the oracle pipeline with the product's own release states and times
+    # the oracle pipeline with the product's own release states and times:
+    # the progenitor's backward orbit as run() computes it (DirectNBody with
+    # the progenitor alone, mockstream_generator.py:229-232)
+    from gala_amd.dynamics import DirectNBody
+    from gala_amd.potential import PlummerPotential
     pot = gen.hamiltonian.potential
-    back = pot.integrate_orbit(prog_w0, dt=dt, n_steps=nsteps)
-    w = back.w()[:, ::-1]
+    back = DirectNBody(prog_w0, [PlummerPotential(m=prog_mass, b=b)],
+                       external_potential=pot).integrate_orbit(Integrator="leapfrog", dt=dt,
+                                                               n_steps=nsteps)
+    w = back.w()[:, ::-1, 0]
     full_t = back.t[::-1]
     rel_t = np.unique(full_t)
     counts = np.full(len(rel_t), 2, dtype=np.int32)
@@ -203,7 +210,8 @@ def test_dop853_release_kernel_matches_oracle():
     keep = [dev.to_dev(pw), dev.to_dev(time), dev.to_dev(ns, dtype=t.int32),
             dev.to_dev(off, dtype=t.int64), dev.to_dev(sw)]
     lib = _lib.load()
-    _lib.check(lib.gamd_mockstream_dop853(ext.c_handle, prog.c_handle, keep[0].data_ptr(), nrel,
+    _lib.check(lib.gamd_mockstream_dop853(ext.c_handle, _lib.body_array([prog.c_handle]), 1,
+                                          keep[0].data_ptr(), nrel,
                                           keep[1].data_ptr(), keep[2].data_ptr(),
                                           keep[3].data_ptr(), keep[4].data_ptr(), 0.0, 5.0,
                                           1e-10, 1e-10, 0, out.data_ptr(), pf.data_ptr(),

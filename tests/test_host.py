@@ -44,10 +44,21 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     assert lib.gamd_dop853(None, None, 1, None, None, None, 1e-8, 1e-10, 0, 0, 0.0, None, 0, None,
                            None, None, None, None, 0, 0, None) == -1
     assert lib.gamd_ruth4(None, None, 1, 0.0, 1.0, 1, 1, None, None, None) == -1
-    assert lib.gamd_mockstream_dop853(None, None, None, 1, None, None, None, None, 0.0, 1.0,
+    assert lib.gamd_mockstream_dop853(None, None, 1, None, 1, None, None, None, None, 0.0, 1.0,
                                       1e-10, 1e-10, 0, None, None, None, None, None) == -1
-    assert lib.gamd_mockstream_leapfrog(None, None, None, 0, 0.0, 1.0, None, None, 0, 0.0, None,
-                                        None) == -1
+    assert lib.gamd_mockstream_leapfrog(None, None, 1, None, 0, 0.0, 1.0, None, None, 0, 0.0,
+                                        None, None) == -1
+    assert lib.gamd_nbody_leapfrog(None, None, 0, None, 1, 0.0, 1.0, 1, 1, None, None,
+                                   None) == -1
+    assert lib.gamd_nbody_dop853(None, None, 0, None, 1, 0.0, 1.0, 1.0, 1e-10, 1e-10, 0, -1,
+                                 0.0, None, 0, None, None, None, None, None) == -1
+    # more massive bodies than the kernels carry, or more bodies than a wave holds
+    many = _lib.body_array([None] * (_lib.MAX_MASSIVE + 1))
+    assert lib.gamd_nbody_leapfrog(None, many, _lib.MAX_MASSIVE + 1, None, 16, 0.0, 1.0, 1, 1,
+                                   None, None, None) == -1
+    assert lib.gamd_nbody_dop853(None, many, _lib.MAX_MASSIVE + 1, None, 16, 0.0, 1.0, 1.0,
+                                 1e-10, 1e-10, 0, -1, 0.0, None, 0, None, None, None, None,
+                                 None) == -1
     assert lib.gamd_potential_destroy(None) == 0
 
 
