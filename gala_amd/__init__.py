@@ -24,6 +24,9 @@ _EXPORTS = {
 
 
 def __getattr__(name):
+    if name == "io":  # potential YAML files
+        import importlib
+        return importlib.import_module(".io", __name__)
     for mod, names in _EXPORTS.items():
         if name in names:
             import importlib
