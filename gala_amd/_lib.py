@@ -60,6 +60,7 @@ SIGNATURES = {
                            _vp, _vp, _vp, _vp, _vp, _i64, _int, _vp]),
     "gamd_nbody_leapfrog": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp,
                                    _vp]),
+    "gamd_nbody_ruth4": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp, _vp]),
     "gamd_nbody_dop853": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _d, _d, _d, _i64, _i64, _d,
                                  _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "gamd_mockstream_leapfrog": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _vp, _vp, _sz, _d,

@@ -1,7 +1,8 @@
 // gamd_mockstream.hip -- Leapfrog N-body systems and batched mock-stream
 // release-and-integrate.
 //
-// C ABI: gamd_nbody_leapfrog, gamd_mockstream_leapfrog (include/gala_amd.h).
+// C ABI: gamd_nbody_leapfrog, gamd_nbody_ruth4, gamd_mockstream_leapfrog
+// (include/gala_amd.h).
 //
 // gamd_nbody_leapfrog restates leapfrog_integrate_nbody (integrate/
 // cyintegrators/leapfrog.pyx:161-257): half kicks for every body
@@ -27,6 +28,7 @@
 // identical in every lane, so each lane reproduces the reference's bodies
 // exactly, and the uneven per-release lengths (0 .. nfull-1 steps) become
 // independent lanes of one launch.
+#include <math.h>
 #include <string.h>
 
 #include "gamd_host.hpp"
@@ -202,6 +204,102 @@ __global__ __launch_bounds__(64) void k_nbody_leapfrog(const DevPot *__restrict_
     if (wfinal) store(wfinal, 1, 0);
 }
 
+// Ruth4 coefficients (ruth4.pyx:67-80, the same in ruth4_integrate_nbody)
+struct Ruth4Coef {
+    double c[4], d[4];
+};
+
+// one c_ruth4_step_nbody (ruth4.pyx:119-136) of the position x, velocity v
+// against the bodies' current positions: four kick-then-drift substeps.  The
+// first kick has d_0 = 0 and adds exactly zero wherever the force is finite,
+// so its force evaluation is skipped.
+template <bool SCF, int KMAX>
+__device__ __forceinline__ void ruth4_substeps(const DevPot &E, const BodyTable &bt, int self,
+                                               const double (&bx)[KMAX][3], double dt,
+                                               const Ruth4Coef &cf, double (&x)[3],
+                                               double (&v)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[c] = x[c] + cf.c[0] * v[c] * dt;
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        double g[3];
+        nbody_grad<SCF, KMAX>(E, bt, self, bx, x[0], x[1], x[2], g[0], g[1], g[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c] = v[c] - cf.d[k] * g[c] * dt;
+            x[c] = x[c] + cf.c[k] * v[c] * dt;
+        }
+    }
+}
+
+// ruth4_integrate_nbody (ruth4.pyx:138-261): per step, body by body in index
+// order, all four substeps of one body before the next (bodies below i have
+// already moved a full step when body i moves); lane l carries the bodies
+// and test particle K + l.  Lane 0 writes the bodies.
+template <bool SCF, bool SAVE, int KMAX>
+__global__ __launch_bounds__(64) void k_nbody_ruth4(const DevPot *__restrict__ Ep, BodyTable bt,
+                                                    const double *__restrict__ w0, int64_t n,
+                                                    double dt, int64_t n_steps, int64_t stride,
+                                                    int64_t nsave, Ruth4Coef cf,
+                                                    double *__restrict__ wout,
+                                                    double *__restrict__ wfinal) {
+    const DevPot &E = *Ep;
+    const int K = bt.n;
+    const int64_t ntest = n - K;
+    const int64_t lane = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (lane >= (ntest > 0 ? ntest : 1)) return;
+    const bool lead = lane == 0;
+    const bool has_p = ntest > 0;
+    const int64_t col = K + lane;
+    double bx[KMAX][3], bv[KMAX][3];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bx[j][c] = j < K ? w0[c * n + j] : 0.0;
+            bv[j][c] = j < K ? w0[(3 + c) * n + j] : 0.0;
+        }
+    }
+    double px[3] = {0.0, 0.0, 0.0}, pv[3] = {0.0, 0.0, 0.0};
+    if (has_p) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            px[c] = w0[c * n + col];
+            pv[c] = w0[(3 + c) * n + col];
+        }
+    }
+    auto store = [&](double *dst, int64_t so, int64_t row) {
+        if (lead) {
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) {
+                if (j < K) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        dst[(c * so + row) * n + j] = bx[j][c];
+                        dst[((3 + c) * so + row) * n + j] = bv[j][c];
+                    }
+                }
+            }
+        }
+        if (has_p) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                dst[(c * so + row) * n + col] = px[c];
+                dst[((3 + c) * so + row) * n + col] = pv[c];
+            }
+        }
+    };
+    if (SAVE) store(wout, nsave, 0);
+    for (int64_t s = 1; s <= n_steps; s++) {
+#pragma unroll
+        for (int i = 0; i < KMAX; i++)
+            if (i < K) ruth4_substeps<SCF, KMAX>(E, bt, i, bx, dt, cf, bx[i], bv[i]);
+        if (has_p) ruth4_substeps<SCF, KMAX>(E, bt, -1, bx, dt, cf, px, pv);
+        if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
+    }
+    if (wfinal) store(wfinal, 1, 0);
+}
+
 // mockstream_leapfrog's release batches: lane i = particle i with its own
 // copy of the bodies from their states at its release.
 template <bool SCF, int KMAX>
@@ -282,6 +380,47 @@ extern "C" int gamd_nbody_leapfrog(const gamd_potential *ext, const gamd_potenti
     hipLaunchKernelGGL(kern, dim3(grid_for((size_t)lanes, block)), dim3(block), 0, as_stream(stream),
                        ext->ddev, bt, w0, (int64_t)n, dt, n_steps, w_out ? save_stride : 1, nsave,
                        w_out, w_final);
+    return launch_status();
+}
+
+extern "C" int gamd_nbody_ruth4(const gamd_potential *ext, const gamd_potential *const *bodies,
+                                int nbody, const double *w0, size_t n, double t0, double dt,
+                                int64_t n_steps, int64_t save_stride, double *w_out,
+                                double *w_final, void *stream) {
+    (void)t0;  // static potentials only: the time argument of c_gradient is unused
+    BodyTable bt;
+    if (make_body_table(bodies, nbody, bt) != GAMD_OK) return GAMD_ERR_INVALID;
+    if (!ext || !w0 || n_steps < 0 || (w_out && save_stride <= 0) || (!w_out && !w_final) ||
+        (size_t)nbody > n)
+        return GAMD_ERR_INVALID;
+    if (n == 0) return GAMD_OK;
+    // ruth4.pyx:174-188
+    const double two_13 = pow(2.0, 1.0 / 3.0);
+    Ruth4Coef cf;
+    cf.c[0] = 1.0 / (2.0 * (2.0 - two_13));
+    cf.c[1] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cf.c[2] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cf.c[3] = 1.0 / (2.0 * (2.0 - two_13));
+    cf.d[0] = 0.0;
+    cf.d[1] = 1.0 / (2.0 - two_13);
+    cf.d[2] = -two_13 / (2.0 - two_13);
+    cf.d[3] = 1.0 / (2.0 - two_13);
+    const int64_t nsave = w_out ? n_steps / save_stride + 1 : 0;
+    const int64_t lanes = (int64_t)n > nbody ? (int64_t)n - nbody : 1;
+    const unsigned block = 64;
+    const bool scf = ext->dev.has_scf || bt.scf;
+    auto pick = [&](auto k1, auto k4, auto k8) { return nbody <= 1 ? k1 : nbody <= 4 ? k4 : k8; };
+    auto kern = w_out ? (scf ? pick(k_nbody_ruth4<true, true, 1>, k_nbody_ruth4<true, true, 4>,
+                                    k_nbody_ruth4<true, true, 8>)
+                             : pick(k_nbody_ruth4<false, true, 1>, k_nbody_ruth4<false, true, 4>,
+                                    k_nbody_ruth4<false, true, 8>))
+                      : (scf ? pick(k_nbody_ruth4<true, false, 1>, k_nbody_ruth4<true, false, 4>,
+                                    k_nbody_ruth4<true, false, 8>)
+                             : pick(k_nbody_ruth4<false, false, 1>, k_nbody_ruth4<false, false, 4>,
+                                    k_nbody_ruth4<false, false, 8>));
+    hipLaunchKernelGGL(kern, dim3(grid_for((size_t)lanes, block)), dim3(block), 0, as_stream(stream),
+                       ext->ddev, bt, w0, (int64_t)n, dt, n_steps, w_out ? save_stride : 1, nsave,
+                       cf, w_out, w_final);
     return launch_status();
 }
 

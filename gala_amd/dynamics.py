@@ -296,8 +296,9 @@ class DirectNBody:
     (core.py:212-227), and returns the orbits in that order.  Integrators:
     DOP853 (the default; direct_nbody_dop853, nbody.pyx:30-115: all bodies one
     system with a shared adaptive step, dense output on t, h0 = t[1] - t[0],
-    stiffness test off) and Leapfrog (leapfrog_integrate_nbody,
-    leapfrog.pyx:161-257).  Limits: at most 8 massive bodies; DOP853 takes at
+    stiffness test off), Leapfrog (leapfrog_integrate_nbody,
+    leapfrog.pyx:161-257) and Ruth4 (ruth4_integrate_nbody,
+    ruth4.pyx:138-261).  Limits: at most 8 massive bodies; DOP853 takes at
     most 64 bodies in all."""
 
     def __init__(self, w0, particle_potentials, external_potential=None, frame=None, units=None,
@@ -331,7 +332,7 @@ class DirectNBody:
         return w0, [self.particle_potentials[i] for i in massive]
 
     def integrate_orbit(self, Integrator=None, Integrator_kwargs=None, **time_spec):
-        from .integrate import nbody_dop853, nbody_leapfrog
+        from .integrate import Ruth4Integrator, nbody_dop853, nbody_leapfrog, nbody_ruth4
         Integrator = get_integrator(Integrator or DOPRI853Integrator)
         t = parse_time_specification(self.units, **time_spec)
         w0, bodies = self._reorganized()
@@ -352,8 +353,11 @@ class DirectNBody:
         elif Integrator is LeapfrogIntegrator:
             wout, wfin = nbody_leapfrog(self.external_potential, bodies, w0, t[1] - t[0],
                                         len(t) - 1, save=self.save_all)
+        elif Integrator is Ruth4Integrator:
+            wout, wfin = nbody_ruth4(self.external_potential, bodies, w0, t[1] - t[0],
+                                     len(t) - 1, save=self.save_all)
         else:
-            raise NotImplementedError("N-body integration supports Leapfrog and DOP853")
+            raise NotImplementedError("N-body integration supports DOP853, Leapfrog and Ruth4")
         if self.save_all:
             w = dev.to_host(wout)  # (6, ntimes, n)
             return Orbit(pos=w[:3], vel=w[3:], t=t, hamiltonian=self.H)

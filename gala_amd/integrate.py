@@ -14,7 +14,7 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
-           "nbody_leapfrog", "nbody_dop853", "integrate_hamiltonian"]
+           "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
 
 
 class Integrator:
@@ -199,6 +199,21 @@ def nbody_leapfrog(potential, body_potentials, w0, dt, n_steps, save_stride=1, s
     _lib.check(lib.gamd_nbody_leapfrog(potential.c_handle, arr, K, w0.data_ptr(), n, float(t0),
                                        float(dt), int(n_steps), int(save_stride), dev.ptr(wout),
                                        wfin.data_ptr(), dev.stream_ptr()), "gamd_nbody_leapfrog")
+    return wout, wfin
+
+
+def nbody_ruth4(potential, body_potentials, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
+    """ruth4_integrate_nbody on device (ruth4.pyx:138-261); arguments and
+    results as nbody_leapfrog."""
+    lib = _lib.load()
+    w0 = dev.to_dev(w0)
+    n = w0.shape[1]
+    arr, K = _bodies(body_potentials)
+    wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, n)) if save else None
+    wfin = dev.empty((6, n))
+    _lib.check(lib.gamd_nbody_ruth4(potential.c_handle, arr, K, w0.data_ptr(), n, float(t0),
+                                    float(dt), int(n_steps), int(save_stride), dev.ptr(wout),
+                                    wfin.data_ptr(), dev.stream_ptr()), "gamd_nbody_ruth4")
     return wout, wfin
 
 

@@ -165,6 +165,14 @@ int gamd_nbody_leapfrog(const gamd_potential *ext, const gamd_potential *const *
                         const double *w0, size_t n, double t0, double dt, int64_t n_steps,
                         int64_t save_stride, double *w_out, double *w_final, void *stream);
 
+/* Replaces ruth4_integrate_nbody (ruth4.pyx:138-261): arguments and layouts
+ * as gamd_nbody_leapfrog.  Within a step each body takes all four
+ * kick-then-drift substeps (c_ruth4_step_nbody, :119-136) before the next
+ * body moves, in index order; test particles move last. */
+int gamd_nbody_ruth4(const gamd_potential *ext, const gamd_potential *const *bodies, int nbody,
+                     const double *w0, size_t n, double t0, double dt, int64_t n_steps,
+                     int64_t save_stride, double *w_out, double *w_final, void *stream);
+
 /* Replaces direct_nbody_dop853 (dynamics/nbody/nbody.pyx:30-115): the n
  * bodies form ONE 6n-dimensional system (body-major state, the reference's
  * (n, 6) rows) with one shared adaptive step and Fwrapper_direct_nbody

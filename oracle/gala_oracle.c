@@ -977,6 +977,46 @@ void orc_nbody_leapfrog(const orc_pot *ext, const orc_pot *const *bodies, int nb
     free(vh);
 }
 
+/* ruth4_integrate_nbody (ruth4.pyx:138-261): per step, body by body in index
+ * order, the four kick-then-drift substeps of c_ruth4_step_nbody
+ * (:119-136), each kick from the gradient against the others' current
+ * positions.  Layouts as orc_nbody_leapfrog. */
+void orc_nbody_ruth4(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                     const double *w0, int n, const double *t, size_t ntimes, int save_all,
+                     double *out) {
+    const double dt = t[1] - t[0];
+    const double two_13 = pow(2.0, 1.0 / 3.0);
+    double cs[4], ds[4];
+    cs[0] = 1.0 / (2.0 * (2.0 - two_13));
+    cs[1] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cs[2] = (1.0 - two_13) / (2.0 * (2.0 - two_13));
+    cs[3] = 1.0 / (2.0 * (2.0 - two_13));
+    ds[0] = 0.0;
+    ds[1] = 1.0 / (2.0 - two_13);
+    ds[2] = -two_13 / (2.0 - two_13);
+    ds[3] = 1.0 / (2.0 - two_13);
+    double *w = (double *)malloc(sizeof(double) * 6 * (n > 0 ? n : 1));
+    memcpy(w, w0, sizeof(double) * 6 * n);
+    if (save_all) memcpy(out, w0, sizeof(double) * 6 * n);
+    for (size_t j = 1; j < ntimes; j++) {
+        for (int i = 0; i < n; i++) {
+            double *x = &w[6 * i], *v = &w[6 * i + 3];
+            for (int s = 0; s < 4; s++) {
+                double g[3];
+                nbody_grad(ext, bodies, nbody, i, x, w, g);
+                for (int k = 0; k < 3; k++) {
+                    v[k] = v[k] - ds[s] * g[k] * dt;
+                    x[k] = x[k] + cs[s] * v[k] * dt;
+                }
+            }
+            if (save_all)
+                for (int k = 0; k < 6; k++) out[(j * n + i) * 6 + k] = w[6 * i + k];
+        }
+    }
+    if (!save_all) memcpy(out, w, sizeof(double) * 6 * n);
+    free(w);
+}
+
 /* direct_nbody_dop853 (nbody.pyx:30-115) through dop853_helper
  * (dop853.pyx:90-193): the n bodies (n, 6) are one system with
  * Fwrapper_direct_nbody forces, integrated over t with h = t[1] - t[0],

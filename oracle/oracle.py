@@ -73,6 +73,8 @@ def lib(fast=False):
                                             C.c_long, dp, dp, ip, lp]
         L.orc_nbody_leapfrog.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
                                          C.c_int, dp]
+        L.orc_nbody_ruth4.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                      C.c_int, dp]
         L.orc_nbody_dop853.restype = C.c_int
         L.orc_nbody_dop853.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
                                        C.c_double, C.c_double, C.c_long, C.c_double, C.c_int, dp,
@@ -114,6 +116,8 @@ def ref_lib():
                                        C.c_double, C.c_double, C.c_long, C.c_double, C.c_int, dp]
         L.ref_nbody_leapfrog.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
                                          C.c_int, dp]
+        L.ref_nbody_ruth4.argtypes = [P, C.c_void_p, C.c_int, dp, C.c_int, dp, C.c_size_t,
+                                      C.c_int, dp]
         L.ref_dop853.argtypes = [P, dp, C.c_size_t, dp, C.c_size_t, C.c_double, C.c_double,
                                  C.c_long, C.c_long, C.c_double, C.c_int, C.c_int, dp, ip]
         _LIBS["ref"] = L
@@ -332,6 +336,19 @@ def nbody_leapfrog(ext_components, bodies, w0, t, save_all=True, ref=False):
     out = np.zeros((len(t), n, 6) if save_all else (n, 6))
     L = ref_lib() if ref else lib()
     fn = L.ref_nbody_leapfrog if ref else L.orc_nbody_leapfrog
+    fn(C.byref(P), arr, K, w0, n, t, len(t), int(save_all), out.reshape(-1))
+    return out
+
+
+def nbody_ruth4(ext_components, bodies, w0, t, save_all=True, ref=False):
+    """ruth4_integrate_nbody (ruth4.pyx:138-261); layouts as nbody_leapfrog."""
+    P = make_pot(ext_components)
+    arr, K, keep = _bodies(bodies)
+    w0 = _c(w0).reshape(-1, 6)
+    n = w0.shape[0]
+    t = _c(t)
+    out = np.zeros((len(t), n, 6) if save_all else (n, 6))
+    fn = ref_lib().ref_nbody_ruth4 if ref else lib().orc_nbody_ruth4
     fn(C.byref(P), arr, K, w0, n, t, len(t), int(save_all), out.reshape(-1))
     return out
 

@@ -14,6 +14,7 @@
  * DOP853 (dopcor + dense output + Fwrapper_T/hamiltonian_gradient_T), driven
  * the way dop853.pyx:90-250 drives it (that Cython file cannot be built here).
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
@@ -335,6 +336,44 @@ void ref_nbody_leapfrog(const orc_pot *ext, const orc_pot *const *bodies, int nb
             for (int k = 0; k < 3; k++) {
                 w[6 * i + 3 + k] = vh[3 * i + k] - grad[k] * dt / 2.;
                 vh[3 * i + k] = vh[3 * i + k] - grad[k] * dt;
+            }
+            if (save_all)
+                for (int k = 0; k < 6; k++) out[(j * n + i) * 6 + k] = w[6 * i + k];
+        }
+    }
+    if (!save_all) memcpy(out, w.data(), sizeof(double) * 6 * n);
+    destroy(re);
+}
+
+// ruth4_integrate_nbody (ruth4.pyx:138-261): the Cython loops
+// (c_ruth4_step_nbody, :119-136) restated around gala's compiled c_gradient
+// and c_nbody_gradient_symplectic.  out (ntimes, n, 6) when save_all, else
+// (n, 6).
+void ref_nbody_ruth4(const orc_pot *ext, const orc_pot *const *bodies, int nbody,
+                     const double *w0, int n, const double *t, size_t ntimes, int save_all,
+                     double *out) {
+    RefPot *re = build(ext);
+    RefBodies rb(bodies, nbody, n);
+    const double dt = t[1] - t[0];
+    const double two_13 = pow(2., 1. / 3.);
+    const double cs[4] = {1. / (2. * (2. - two_13)), (1. - two_13) / (2. * (2. - two_13)),
+                          (1. - two_13) / (2. * (2. - two_13)), 1. / (2. * (2. - two_13))};
+    const double ds[4] = {0., 1. / (2. - two_13), -two_13 / (2. - two_13), 1. / (2. - two_13)};
+    std::vector<double> w(w0, w0 + 6 * n);
+    if (save_all) memcpy(out, w0, sizeof(double) * 6 * n);
+    double grad[3];
+    for (size_t j = 1; j < ntimes; j++) {
+        for (int i = 0; i < n; i++) {
+            double *wi = &w[6 * i];
+            for (int s = 0; s < 4; s++) {
+                for (int k = 0; k < 3; k++) grad[k] = 0.;
+                c_gradient(re->cp, 1, t[j], wi, grad);
+                c_nbody_gradient_symplectic(rb.pots.data(), t[j], wi, w.data(), nbody, i, 3,
+                                            grad);
+                for (int k = 0; k < 3; k++) {
+                    wi[3 + k] = wi[3 + k] - ds[s] * grad[k] * dt;
+                    wi[k] = wi[k] + cs[s] * wi[3 + k] * dt;
+                }
             }
             if (save_all)
                 for (int k = 0; k < 6; k++) out[(j * n + i) * 6 + k] = w[6 * i + k];

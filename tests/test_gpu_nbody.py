@@ -1,7 +1,7 @@
 """N-body systems on the GPU against the oracle (itself bit-for-bit equal to
 gala's compiled dop853 + Fwrapper_direct_nbody and c_nbody_gradient_symplectic,
 tests/test_oracle_nbody.py): DirectNBody with several massive bodies under
-DOP853 and Leapfrog, and mock streams with a massive perturber.
+DOP853, Leapfrog and Ruth4, and mock streams with a massive perturber.
 
 Tolerances: DOP853 states within 1e-8 relative, Leapfrog within 1e-10, or
 100x the oracle's own sensitivity to a one-ulp change of the initial state
@@ -110,6 +110,34 @@ def test_direct_nbody_leapfrog_matches_oracle(n_massive, n_test):
     fin = DirectNBody(w.T, pots, external_potential=mw_product(), save_all=False)
     wf = fin.integrate_orbit(t=t, Integrator=LeapfrogIntegrator).w()
     assert np.array_equal(wf, got[:, -1])
+
+
+@pytest.mark.parametrize("n_massive", [2, 5])
+def test_direct_nbody_ruth4_matches_oracle(n_massive):
+    """Ruth4 N-body (all four substeps of a body before the next body moves)
+    with bodies in registers (2) and spilled (5)."""
+    from gala_amd.dynamics import DirectNBody
+    from gala_amd.integrate import Ruth4Integrator
+    from gala_amd.potential import PlummerPotential
+    rng = np.random.default_rng(7)
+    w = _system(8 + n_massive - 2)
+    w[2:n_massive, :3] += rng.normal(0, 5.0, (n_massive - 2, 3))
+    lmc, lmc_o = _lmc()
+    pots = [PlummerPotential(m=2e8, b=0.5), lmc] + [PlummerPotential(m=1e9, b=0.3)] * \
+        (n_massive - 2)
+    bodies = [[O.plummer(2e8, 0.5)], lmc_o] + [[O.plummer(1e9, 0.3)]] * (n_massive - 2)
+    pots += [None] * (w.shape[0] - n_massive)
+    t = np.arange(0.0, 401.0, 1.0)
+    got = DirectNBody(w.T, pots, external_potential=mw_product()).integrate_orbit(
+        t=t, Integrator=Ruth4Integrator).w()
+    ref = O.nbody_ruth4(mw_oracle(), bodies, w, t)
+    pert = w.copy()
+    pert[:, 0] = np.nextafter(pert[:, 0], np.inf)
+    alt = O.nbody_ruth4(mw_oracle(), bodies, pert, t)
+    ref_w, alt_w = np.moveaxis(ref, -1, 0), np.moveaxis(alt, -1, 0)
+    err = state_rel_err(got, ref_w)
+    print(f"nbody ruth4 K={n_massive}: max rel err {err.max():.3e}")
+    assert np.all(err <= _gate(ref_w, alt_w, 1e-10))
 
 
 def _release_inputs(nrel=40, seed=4):

@@ -50,6 +50,7 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
                                         None, None) == -1
     assert lib.gamd_nbody_leapfrog(None, None, 0, None, 1, 0.0, 1.0, 1, 1, None, None,
                                    None) == -1
+    assert lib.gamd_nbody_ruth4(None, None, 0, None, 1, 0.0, 1.0, 1, 1, None, None, None) == -1
     assert lib.gamd_nbody_dop853(None, None, 0, None, 1, 0.0, 1.0, 1.0, 1e-10, 1e-10, 0, -1,
                                  0.0, None, 0, None, None, None, None, None) == -1
     # more massive bodies than the kernels carry, or more bodies than a wave holds

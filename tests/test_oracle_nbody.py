@@ -1,5 +1,6 @@
 """The oracle's N-body restatements (direct_nbody_dop853,
-leapfrog_integrate_nbody, mock streams with perturbers) pinned three ways:
+leapfrog_integrate_nbody, ruth4_integrate_nbody, mock streams with
+perturbers) pinned three ways:
 
 * bit for bit against gala's own compiled dop853() with Fwrapper_direct_nbody
   and c_nbody_acceleration, and against gala's compiled c_gradient +
@@ -71,6 +72,17 @@ def test_nbody_leapfrog_bitwise_against_reference():
 
 
 @needs_ref
+def test_nbody_ruth4_bitwise_against_reference():
+    ext = O.milky_way_like()
+    w0 = _w0(2)
+    t = np.arange(0.0, 401.0, 1.0)
+    a = O.nbody_ruth4(ext, _bodies(), w0, t)
+    b = O.nbody_ruth4(ext, _bodies(), w0, t, ref=True)
+    assert np.array_equal(a, b)
+    assert np.array_equal(O.nbody_ruth4(ext, _bodies(), w0, t, save_all=False), a[-1])
+
+
+@needs_ref
 def test_mockstream_dop853_with_perturber_bitwise():
     """Release systems of progenitor + perturber + particles (nbody = 2)."""
     ext = O.milky_way_like()
@@ -106,6 +118,9 @@ def test_one_massive_body_is_an_orbit():
     c = O.nbody_leapfrog(ext, [_bodies()[0]], w0, tl)
     d = O.leapfrog(ext, w0.T, tl)
     assert np.array_equal(c[:, 0, :], d[:, :, 0].T)
+    e = O.nbody_ruth4(ext, [_bodies()[0]], w0, tl)
+    f = O.ruth4(ext, w0.T, tl)
+    assert np.array_equal(e[:, 0, :], f[:, :, 0].T)
 
 
 def test_massless_perturber_leaves_leapfrog_stream_unchanged():
