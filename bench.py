@@ -66,7 +66,7 @@ def cpu_baseline_dop853(w0, T, stride):
 def bench_dop853(args):
     import torch
     from gala_amd import _device as dev
-    from gala_amd import roofline, workloads
+    from gala_amd import autotune, roofline, workloads
     from gala_amd.integrate import dop853
     from gala_amd.potential import MilkyWayLikePotential
 
@@ -81,10 +81,20 @@ def bench_dop853(args):
     h0d = torch.ones(n, dtype=torch.float64, device="cuda")
     ordd = dev.to_dev(order, dtype=torch.int32)
     torch.cuda.synchronize()
+    block, n_tail, tuned = args.block, args.n_tail, None
+    if block == 0 or n_tail is None:
+        # the project's autotuner: whole passes of this workload per candidate
+        # launch shape (results are identical for every candidate)
+        tuned = autotune.tune_dop853(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, order=ordd,
+                                     blocks=autotune.BLOCKS if block == 0 else (block,),
+                                     tails=autotune.TAILS if n_tail is None else (n_tail,),
+                                     log=log)
+        block, n_tail = tuned["block"], tuned["n_tail"]
+        log(f"autotuned: block {block}, n_tail {n_tail}")
 
     def one():
         return dop853(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, order=ordd,
-                      n_tail=args.n_tail, block=args.block)
+                      n_tail=n_tail, block=block)
 
     for i in range(args.warmup):
         one()
@@ -127,7 +137,9 @@ def bench_dop853(args):
                    "rtol": 1e-8, "atol": 1e-10, "T": "logU(0.1, 10) Gyr per orbit",
                    "r0": "logU(1, 200) kpc", "step": "one integration of every orbit",
                    "orbit_step": "accepted DOP853 step", "parallelism": "single GPU",
-                   "queue": f"longest first, {args.n_tail} on dedicated 16-lane rows"},
+                   "queue": f"longest first, {n_tail} on dedicated 16-lane rows",
+                   "block": block,
+                   "autotune_ms": None if tuned is None else tuned["trials_ms"]},
         "accepted_steps": naccpt, "rejected_steps": nrejct,
         "max_attempts_one_orbit": int(stats[:, 0].max()),
         "fevals_per_s": nfcn / (ms_per_step / 1000.0),
@@ -363,8 +375,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="composite_dop853")
     ap.add_argument("--n", type=int, default=0, help="override the orbit count (testing)")
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--n-tail", type=int, default=1024,
+    ap.add_argument("--block", type=int, default=0, help="DOP853 workgroup size (0: autotune)")
+    ap.add_argument("--n-tail", type=int, default=None,
                     help="longest orbits integrated on dedicated 16-lane rows from the start")
     ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU baseline sample: every k-th orbit")

@@ -866,7 +866,8 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
         return GAMD_ERR_INVALID;
     if (n_tail < 0) return GAMD_ERR_INVALID;
     if (block == 0) block = 256;
-    if (block < 64 || block > 1024 || block % 64) return GAMD_ERR_INVALID;
+    // whole waves, at most the kernel's launch bound
+    if (block < 64 || block > 256 || block % 64) return GAMD_ERR_INVALID;
     if (nmax == 0) nmax = 1000000;                   // dop853.cpp:673-674
     else if (nmax < 0) return GAMD_ERR_INVALID;      // :675-679
     if (nstiff == 0) nstiff = 1000;                  // :691-692

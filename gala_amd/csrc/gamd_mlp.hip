@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256) void k_mlp_forward_train(const float *__restri
 //   C[m][n] = sum_k A[m][k] B[k][n], A[m][k] = A[m*sam + k*sak], B[k][n] = B[k*sbk + n*sbn]
 // EPI 0: C (FP32, ldc) = acc.  EPI 1: out (BF16, ldc) = acc * silu'(Zprev[m][n]).
 // Tile 64 x 64 per workgroup (4 waves of 32 x 32); M, N multiples of 64, K of 32.
-template <int EPI>
+// KC: both operands contiguous along k (sak = sbk = 1), read as 16-byte vectors.
+template <int EPI, bool KC = false>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const __bf16 *__restrict__ A,
                                               int64_t sam, int64_t sak,
                                               const __bf16 *__restrict__ B, int64_t sbk,
@@ -302,14 +303,22 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const __bf16 
 #pragma unroll
         for (int i = 0; i < 2; i++) {
             const int64_t row = m0 + 16 * i + li;
+            if (KC) {
+                a[i] = *reinterpret_cast<const bf16x8 *>(A + row * sam + k0 + lk);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; e++) a[i][e] = A[row * sam + (int64_t)(k0 + lk + e) * sak];
+                for (int e = 0; e < 8; e++) a[i][e] = A[row * sam + (int64_t)(k0 + lk + e) * sak];
+            }
         }
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int64_t col = n0 + 16 * j + li;
+            if (KC) {
+                b[j] = *reinterpret_cast<const bf16x8 *>(B + k0 + lk + col * sbn);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; e++) b[j][e] = B[(int64_t)(k0 + lk + e) * sbk + col * sbn];
+                for (int e = 0; e < 8; e++) b[j][e] = B[(int64_t)(k0 + lk + e) * sbk + col * sbn];
+            }
         }
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -397,32 +406,69 @@ __global__ void k_outer_reduce(const __bf16 *__restrict__ X, int nx, const float
         for (int o = 0; o < ny; o++) atomicAdd(&out[j * ny + o], red[0][o]);
 }
 
-// bias gradients: gb[j] += sum_m dZ[m][j] (dZ BF16, M x 256).  Grid (256/64... ) simple.
-__global__ void k_colsum(const __bf16 *__restrict__ dZ, int64_t M, float *__restrict__ gb) {
-    const int j = threadIdx.x;  // 256 threads, one column each
-    const int64_t chunk = (M + gridDim.x - 1) / gridDim.x;
-    const int64_t m_lo = blockIdx.x * chunk;
-    const int64_t m_hi = m_lo + chunk < M ? m_lo + chunk : M;
-    float s = 0.f;
-    for (int64_t m = m_lo; m < m_hi; m++) s += frombf(dZ[m * H + j]);
-    atomicAdd(&gb[j], s);
+// Column reductions over the M rows of a BF16 (M x 256) matrix, one block per
+// CS_ROWS rows: thread t reads 8 adjacent columns (one 16-byte load) of rows
+// t / 32, t / 32 + 8, ...; the 8 row groups are summed in LDS and each block
+// adds its 256 partial sums with atomics.
+constexpr int CS_ROWS = 128;
+
+// bias gradients: gb[j] += sum_m dZ[m][j]
+__global__ __launch_bounds__(256) void k_colsum(const __bf16 *__restrict__ dZ, int64_t M,
+                                                float *__restrict__ gb) {
+    __shared__ float red[8][H];
+    const int t = threadIdx.x, c8 = 8 * (t & 31), rg = t >> 5;
+    const int64_t r0 = (int64_t)blockIdx.x * CS_ROWS;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int r = rg; r < CS_ROWS; r += 8) {
+        const int64_t m = r0 + r;
+        if (m >= M) break;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ + m * H + c8);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += frombf(v[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[rg][c8 + e] = s[e];
+    __syncthreads();
+    float tot = 0.f;
+#pragma unroll
+    for (int g = 0; g < 8; g++) tot += red[g][t];
+    atomicAdd(&gb[t], tot);
 }
 
 // first-layer weight gradient: gW1[c][j] += sum_m feat[m][c] dZ1[m][j]
-__global__ void k_w1_grad(const float *__restrict__ feat, const __bf16 *__restrict__ dZ1,
-                          int64_t M, float *__restrict__ gW1) {
-    const int j = threadIdx.x;
-    const int64_t chunk = (M + gridDim.x - 1) / gridDim.x;
-    const int64_t m_lo = blockIdx.x * chunk;
-    const int64_t m_hi = m_lo + chunk < M ? m_lo + chunk : M;
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t m = m_lo; m < m_hi; m++) {
-        const float d = frombf(dZ1[m * H + j]);
+__global__ __launch_bounds__(256) void k_w1_grad(const float *__restrict__ feat,
+                                                 const __bf16 *__restrict__ dZ1, int64_t M,
+                                                 float *__restrict__ gW1) {
+    __shared__ float red[8][4][H];
+    const int t = threadIdx.x, c8 = 8 * (t & 31), rg = t >> 5;
+    const int64_t r0 = (int64_t)blockIdx.x * CS_ROWS;
+    float s[4][8];
 #pragma unroll
-        for (int c = 0; c < 4; c++) s[c] += feat[m * 4 + c] * d;
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[c][e] = 0.f;
+    for (int r = rg; r < CS_ROWS; r += 8) {
+        const int64_t m = r0 + r;
+        if (m >= M) break;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(dZ1 + m * H + c8);
+        const f32x4 f = *reinterpret_cast<const f32x4 *>(feat + m * 4);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[c][e] += f[c] * frombf(v[e]);
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) atomicAdd(&gW1[c * H + j], s[c]);
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) red[rg][c][c8 + e] = s[c][e];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float tot = 0.f;
+#pragma unroll
+        for (int g = 0; g < 8; g++) tot += red[g][c][t];
+        atomicAdd(&gW1[c * H + t], tot);
+    }
 }
 
 // BF16 copies of W2..W4: swizzled B-fragment order and row-major
@@ -679,6 +725,7 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
     hipLaunchKernelGGL(k_out_backward_dz, dim3(grid_for(Mi * H, 256)), dim3(256), 0, s, theta, dY,
                        Zs + 3 * Mi * H, Mi, dZa);
     // hidden layers 4, 3, 2: dW_l = H_{l-1}^T dZ_l, db_l, dZ_{l-1} = (dZ_l W_l^T) silu'(Z_{l-1})
+    const unsigned cs_blocks = (unsigned)((Mi + CS_ROWS - 1) / CS_ROWS);
     __bf16 *dZ = dZa, *dZn = dZb;
     const int offW[3] = {OFF_W2, OFF_W3, OFF_W4}, offB[3] = {OFF_B2, OFF_B3, OFF_B4};
     for (int l = 2; l >= 0; l--) {  // l indexes W_{l+2}
@@ -687,8 +734,8 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
         hipLaunchKernelGGL(k_gemm<0>, dim3(H / 64, H / 64, ksplit), dim3(256), 0, s, H, H, (int)Mi, Hin,
                            (int64_t)1, (int64_t)H, dZ, (int64_t)H, (int64_t)1, grad + offW[l],
                            (__bf16 *)nullptr, (int64_t)H, (const __bf16 *)nullptr);
-        hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, dZ, Mi, grad + offB[l]);
-        hipLaunchKernelGGL(k_gemm<1>, dim3((unsigned)(Mi / 64), H / 64), dim3(256), 0, s, (int)Mi, H,
+        hipLaunchKernelGGL(k_colsum, dim3(cs_blocks), dim3(256), 0, s, dZ, Mi, grad + offB[l]);
+        hipLaunchKernelGGL((k_gemm<1, true>), dim3((unsigned)(Mi / 64), H / 64), dim3(256), 0, s, (int)Mi, H,
                            H, dZ, (int64_t)H, (int64_t)1, wrow + (size_t)l * H * H, (int64_t)1,
                            (int64_t)H, (float *)nullptr, dZn, (int64_t)H, Zs + (size_t)l * Mi * H);
         __bf16 *tmp = dZ;
@@ -696,8 +743,8 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
         dZn = tmp;
     }
     // first layer
-    hipLaunchKernelGGL(k_w1_grad, dim3(64), dim3(256), 0, s, feat, dZ, Mi, grad + OFF_W1);
-    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, s, dZ, Mi, grad + OFF_B1);
+    hipLaunchKernelGGL(k_w1_grad, dim3(cs_blocks), dim3(256), 0, s, feat, dZ, Mi, grad + OFF_W1);
+    hipLaunchKernelGGL(k_colsum, dim3(cs_blocks), dim3(256), 0, s, dZ, Mi, grad + OFF_B1);
     if (loss) {
         // sum of the per-block partials into loss[0]
         GAMD_HIP_CHECK(hipMemsetAsync(loss, 0, sizeof(float), s));

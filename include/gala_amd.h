@@ -134,8 +134,10 @@ int gamd_ruth4(const gamd_potential *pot, const double *w0, size_t N, double t0,
  * the tail short).  n_tail >= 0: the first n_tail orbits of that order are
  * integrated from the start by dedicated waves, four orbits per wave, each
  * spread over 16 lanes (capped at half the grid; ignored with dense output);
- * results are identical for every n_tail.  block: threads per workgroup
- * (0 = autotuned). */
+ * results are identical for every n_tail.  block: threads per workgroup,
+ * 64, 128, 192 or 256 (0 = 256); results are identical for every block.
+ * Both are speed knobs only: gala_amd.autotune times them on the caller's
+ * workload. */
 int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
                 const double *t1, const double *h0, double rtol, double atol, int64_t nmax,
                 int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
