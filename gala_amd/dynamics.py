@@ -365,6 +365,33 @@ class DirectNBody:
         return PhaseSpacePosition(w[:3], w[3:], frame=self.frame)
 
 
+def release_counts(orbit_t, t1s, counts):
+    """The release bookkeeping of mockstream_generator.py:272-276: for each
+    release time t1s[k] in order, every grid time with np.isclose(orbit_t,
+    t1s[k]) gets counts[k] (a later k overwrites an earlier one).  Exact and
+    vectorised: candidates come from a binary search of the sorted grid and
+    are confirmed with np.isclose itself."""
+    orbit_t = np.asarray(orbit_t, dtype=np.float64)
+    t1s = np.asarray(t1s, dtype=np.float64)
+    out = np.zeros(len(orbit_t), dtype=int)
+    if len(orbit_t) == 0 or len(t1s) == 0:
+        return out
+    order = np.argsort(orbit_t, kind="stable")
+    ts = orbit_t[order]
+    tol = 1e-8 + 1e-5 * np.abs(t1s)  # np.isclose's default atol + rtol |b|
+    lo = np.maximum(np.searchsorted(ts, t1s - 2 * tol, side="left") - 1, 0)
+    hi = np.minimum(np.searchsorted(ts, t1s + 2 * tol, side="right") + 1, len(ts))
+    width = np.maximum(hi - lo, 0)
+    k = np.repeat(np.arange(len(t1s)), width)
+    pos = np.repeat(lo - np.concatenate([[0], np.cumsum(width)[:-1]]), width) + np.arange(width.sum())
+    hit = np.isclose(ts[pos], t1s[k])
+    last = np.full(len(orbit_t), -1)
+    np.maximum.at(last, order[pos[hit]], k[hit])
+    sel = last >= 0
+    out[sel] = np.asarray(counts)[last[sel]]
+    return out
+
+
 class MockStreamGenerator:
     """Generate a mock stream (mockstream_generator.py:19-372).  Leapfrog
     runs the release-and-integrate loop in the HIP kernel
@@ -445,9 +472,7 @@ class MockStreamGenerator:
                                    release_every=release_every, n_particles=n_particles)
         # release bookkeeping (mockstream_generator.py:272-283)
         unq_t1s, nstream = np.unique(stream_w0.release_time, return_counts=True)
-        all_nstream = np.zeros(len(orbit_t), dtype=int)
-        for t1, n in zip(unq_t1s, nstream):
-            all_nstream[np.isclose(orbit_t, t1)] = n
+        all_nstream = release_counts(orbit_t, unq_t1s, nstream)
         # the bodies' forward trajectory on the full grid (mockstream.pyx:536-549)
         dt = orbit_t[1] - orbit_t[0]
         nfull = len(orbit_t)
@@ -527,9 +552,7 @@ class MockStreamGenerator:
                                    release_every=release_every, n_particles=n_particles)
         # release bookkeeping (mockstream_generator.py:272-292)
         unq_t1s, nstream = np.unique(stream_w0.release_time, return_counts=True)
-        all_nstream = np.zeros(len(orbit_t), dtype=int)
-        for t1, n in zip(unq_t1s, nstream):
-            all_nstream[np.isclose(orbit_t, t1)] = n
+        all_nstream = release_counts(orbit_t, unq_t1s, nstream)
         nstream_idx = np.where(all_nstream != 0)[0]
         if 0 not in nstream_idx:
             nstream_idx = np.insert(nstream_idx, 0, 0)

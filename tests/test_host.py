@@ -180,3 +180,26 @@ def test_no_cpu_fallback_in_product():
             if f.endswith((".py", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower() or f == "__init__.py", f
+
+
+def test_release_counts_matches_isclose_loop():
+    """The vectorised release bookkeeping equals the reference's loop
+    (mockstream_generator.py:272-276) on grids with exact, near-tolerance,
+    overlapping and missing release times."""
+    from gala_amd.dynamics import release_counts
+    rng = np.random.default_rng(0)
+
+    def loop(orbit_t, t1s, counts):
+        out = np.zeros(len(orbit_t), dtype=int)
+        for t1, n in zip(t1s, counts):
+            out[np.isclose(orbit_t, t1)] = n
+        return out
+    grids = [np.arange(4001) * 1.0 - 4000.0, np.sort(rng.uniform(-100, 100, 500)),
+             np.array([0.0, 1e-9, 2e-8, 1.0, 1.0 + 1e-6, 1.00002, 5.0]),
+             np.linspace(0, 1, 11)[::-1]]
+    for g in grids:
+        for t1s in (np.unique(g), np.unique(g[::3]),
+                    np.unique(np.concatenate([g[::2] + rng.normal(0, 1e-5, len(g[::2])),
+                                              g[:3] + 2e-8]))):
+            counts = rng.integers(1, 5, len(t1s))
+            assert np.array_equal(release_counts(g, t1s, counts), loop(g, t1s, counts))
