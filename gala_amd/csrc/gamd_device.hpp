@@ -301,6 +301,13 @@ __device__ __forceinline__ void comp_hessian_analytic(const DevComp &c, double x
 // wave-uniform table read instead of two FP64 divisions per (l, n) term
 static __constant__ double scf_inv_n[GAMD_SCF_MAX_NMAX + 2] = {0.0, 1.0 / 1.0, 1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0, 1.0 / 7.0, 1.0 / 8.0, 1.0 / 9.0, 1.0 / 10.0, 1.0 / 11.0, 1.0 / 12.0, 1.0 / 13.0, 1.0 / 14.0, 1.0 / 15.0, 1.0 / 16.0, 1.0 / 17.0, 1.0 / 18.0, 1.0 / 19.0, 1.0 / 20.0, 1.0 / 21.0};
 
+// Position of (S_nlm, T_nlm) in the device coefficient table: per l, per n,
+// the l + 1 pairs of m = 0..l contiguous, so one n step of the expansion reads
+// one contiguous run of 2 (l + 1) doubles (wide scalar loads).
+__host__ __device__ __forceinline__ int scf_pack_index(int l, int m, int n, int nmax) {
+    return (nmax + 1) * (l * (l + 1) / 2) + n * (l + 1) + m;
+}
+
 template <int NMAX, int LMAX>
 __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double G, double M,
                                            double rs, int nmax_rt, int lmax_rt, double x,
@@ -410,8 +417,8 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
 #pragma unroll
             for (int m = 0; m <= (LMAX >= 0 ? LMAX : GAMD_SCF_MAX_LMAX); m++) {
                 if (m <= l) {
-                    const double S = cfc[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n)];
-                    const double T = cfc[2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n) + 1];
+                    const double S = cfc[2 * scf_pack_index(l, m, n, nmax)];
+                    const double T = cfc[2 * scf_pack_index(l, m, n, nmax) + 1];
                     As[m] = fma(phinl, S, As[m]);
                     Bs[m] = fma(phinl, T, Bs[m]);
                     dAs[m] = fma(dphinl, S, dAs[m]);
@@ -548,7 +555,7 @@ __device__ inline double scf_density(const DevComp &c, double x, double y, doubl
                 c0 = Cn;
                 const double Knl = 0.5 * n * (n + 4 * l + 3) + (l + 1) * (2 * l + 1);
                 const double rnl = Knl * pre * Cn;
-                const int idx = 2 * ((l * (l + 1) / 2 + m) * (nmax + 1) + n);
+                const int idx = 2 * scf_pack_index(l, m, n, nmax);
                 A += rnl * c.coef[idx];
                 B += rnl * c.coef[idx + 1];
             }

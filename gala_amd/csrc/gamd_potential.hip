@@ -172,12 +172,12 @@ int gamd_potential_create(const gamd_component *components, int ncomponents,
             const size_t nlm = (size_t)(lmax + 1) * (lmax + 2) / 2;
             std::vector<double> packed(nlm * (nmax + 1) * 2);
             // gala's index i = m + (lmax+1)(l + (lmax+1) n) (bfe.cpp:160) ->
-            // per (l, m) block, n fastest, (S, T) interleaved
+            // per l, per n, m = 0..l, (S, T) interleaved (scf_pack_index)
             for (int l = 0; l <= lmax; l++)
                 for (int m = 0; m <= l; m++)
                     for (int n = 0; n <= nmax; n++) {
                         const size_t gi = (size_t)m + (lmax + 1) * (l + (size_t)(lmax + 1) * n);
-                        const size_t pi = ((size_t)l * (l + 1) / 2 + m) * (nmax + 1) + n;
+                        const size_t pi = (size_t)scf_pack_index(l, m, n, nmax);
                         packed[2 * pi] = src.S[gi];
                         packed[2 * pi + 1] = src.T[gi];
                     }
