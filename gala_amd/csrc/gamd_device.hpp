@@ -391,7 +391,9 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
         // Gegenbauer recurrences in n
         double c0 = 1.0, c1 = 0.0;  // C_n^a, C_{n-1}^a
         double e0 = 0.0, e1 = 0.0;  // C_{n-1}^{a+1}, C_{n-2}^{a+1}
-#pragma unroll 1
+        // the fixed-size evaluator unrolls n by 10 (measured best for nmax 10;
+        // fully unrolled, the loop spills); the runtime-size one keeps a loop
+#pragma unroll(NMAX >= 0 ? 10 : 1)
         for (int n = 0; n <= nmax; n++) {
             double Cn, En;  // C_n^a(xi), C_{n-1}^{a+1}(xi)
             if (n == 0) {
