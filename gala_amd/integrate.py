@@ -14,7 +14,7 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
-           "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
+           "pilot_order", "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
 
 
 class Integrator:
@@ -138,25 +138,53 @@ def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
     return wout, wfin
 
 
+PILOT_FRACTION = 0.02
+
+
+def pilot_order(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0,
+                hmax=0.0, fraction=PILOT_FRACTION, block=0):
+    """Longest-first queue order for gamd_dop853 from a pilot pass: every
+    orbit integrated over the first `fraction` of its time span with the same
+    tolerances; its step count / fraction estimates the whole run's, and the
+    orbits are ordered by that estimate, descending (device int32 tensor).
+    Eccentric orbits through the nucleus cost many more steps per orbital
+    period than their dynamical time suggests; the pilot measures it."""
+    t = dev.require_gpu()
+    w0 = dev.to_dev(w0)
+    N = w0.shape[1]
+    t0d, t1d = _per_orbit(t0, N), _per_orbit(t1, N)
+    rp = dop853(potential, w0, t0d, t0d + (t1d - t0d) * fraction, h0, rtol=rtol, atol=atol,
+                nmax=nmax, nstiff=nstiff, hmax=hmax, block=block)
+    est = rp["stats"][:, 0].to(t.float64)
+    return t.argsort(est, descending=True, stable=True).to(t.int32)
+
+
+def _per_orbit(v, N):
+    t = dev.torch()
+    if v is None:
+        return None
+    if isinstance(v, t.Tensor):
+        return dev.to_dev(v.expand(N) if v.dim() == 0 else v)
+    return dev.to_dev(np.broadcast_to(np.asarray(v, dtype=np.float64), (N,)))
+
+
 def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0, hmax=0.0,
            t_out=None, order=None, n_tail=0, block=0):
     """Per-orbit adaptive DOP853 on device.  w0 (6, N); t0, t1, h0 per orbit
-    (scalars broadcast); order: queue order; n_tail: the first n_tail orbits
-    of that order run on dedicated 16-lane rows from the start.
+    (scalars broadcast); order: queue order (a permutation, or "pilot" for
+    pilot_order's estimate, computed in the same call); n_tail: the first
+    n_tail orbits of that order run on dedicated 16-lane rows from the start.
     Returns dict(w_final, status, stats[, w_out])."""
     t = dev.require_gpu()
     lib = _lib.load()
     w0 = dev.to_dev(w0)
     N = w0.shape[1]
-
-    def per_orbit(v):
-        if v is None:
-            return None
-        if isinstance(v, t.Tensor):
-            return dev.to_dev(v.expand(N) if v.dim() == 0 else v)
-        return dev.to_dev(np.broadcast_to(np.asarray(v, dtype=np.float64), (N,)))
-
-    t0d, t1d, h0d = per_orbit(t0), per_orbit(t1), per_orbit(h0)
+    if isinstance(order, str):
+        if order != "pilot":
+            raise ValueError(f"unknown order {order!r}")
+        order = pilot_order(potential, w0, t0, t1, h0, rtol=rtol, atol=atol, nmax=nmax,
+                            nstiff=nstiff, hmax=hmax, block=block)
+    t0d, t1d, h0d = _per_orbit(t0, N), _per_orbit(t1, N), _per_orbit(h0, N)
     wfin = dev.empty((6, N))
     status = dev.empty((N,), dtype=t.int32)
     stats = dev.empty((N, 4), dtype=t.int64)
