@@ -54,7 +54,8 @@ struct DopArgs {
     int64_t *stats;
     const int32_t *order;
     unsigned long long *queue;
-    int wide;              // finish the last orbits of a wave in 16-lane rows
+    int wide;              // 0: one lane per orbit to the end; 1: the last <= 4 orbits of a
+                           // wave on 16-lane rows; 2: the last <= 16 on 4-lane quads first
     double uround;         // the "step size too small" threshold (dop853.cpp:349-354)
     int64_t n_tail;        // queue slots [0, n_tail) go to dedicated wide waves
 };
@@ -130,6 +131,7 @@ __device__ double hinit(const DevPot &P, double x, const double *y, double posne
 
 constexpr int WIDE_ROW = 16;   // lanes per orbit in the wide tail (one DPP row)
 constexpr int WIDE_ORBITS = 4; // orbits per wave in the wide tail
+constexpr int QUAD_ORBITS = 16; // orbits per wave in the quad tail (one per 4-lane quad)
 
 template <int SRC>
 __device__ __forceinline__ double row_bcast(double v) {
@@ -219,6 +221,37 @@ struct WideRow {
         double qa[6], qb[6];
         all6(a, qa);
         all6(b, qb);
+        FOR6 {
+            sa += qa[c] * qa[c];
+            sb += qb[c] * qb[c];
+        }
+    }
+    __device__ __forceinline__ double n() const { return 6.0; }
+};
+
+// one orbit per 4-lane quad, lane c < 3 holding components c and c + 3 (the
+// position and velocity along one axis), lane 3 mirroring lane 0.  The stage
+// combinations cost two components' arithmetic instead of six; every lane
+// evaluates the force at the quad's broadcast position.  Sums over
+// components are formed in the one-lane order, so results are bit-identical
+// to one lane per orbit.
+template <bool SCF, int LAY>
+struct QuadPair {
+    static constexpr int NC = 2;
+    int axis;
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[2], double (&f)[2]) const {
+        const double qx = quad_bcast<0>(y[0]), qy = quad_bcast<1>(y[0]), qz = quad_bcast<2>(y[0]);
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(P, qx, qy, qz, gx, gy, gz);
+        f[0] = y[1];
+        f[1] = -(axis == 0 ? gx : (axis == 1 ? gy : gz));
+    }
+    __device__ __forceinline__ void sums(const double (&a)[2], const double (&b)[2], double &sa,
+                                         double &sb) const {
+        const double qa[6] = {quad_bcast<0>(a[0]), quad_bcast<1>(a[0]), quad_bcast<2>(a[0]),
+                              quad_bcast<0>(a[1]), quad_bcast<1>(a[1]), quad_bcast<2>(a[1])};
+        const double qb[6] = {quad_bcast<0>(b[0]), quad_bcast<1>(b[0]), quad_bcast<2>(b[0]),
+                              quad_bcast<0>(b[1]), quad_bcast<1>(b[1]), quad_bcast<2>(b[1])};
         FOR6 {
             sa += qa[c] * qa[c];
             sb += qb[c] * qb[c];
@@ -610,6 +643,77 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
     }
 }
 
+// The quad tail of one wave: the (at most QUAD_ORBITS) orbits still active
+// move to one 4-lane quad each; once at most WIDE_ORBITS remain they move on
+// to 16-lane rows (wide_tail).  Called with every lane of the wave converged;
+// the queue is drained.
+template <bool SCF, int LAY>
+__device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, bool active,
+                                       const DopState &s6, const double (&y6)[6],
+                                       const double (&k16)[6]) {
+    const int lane = threadIdx.x & 63;
+    const int quad = lane >> 2, c = lane & 3;
+    const int axis = c < 3 ? c : 0;
+    unsigned long long m = __ballot(active);
+    for (int i = 0; i < quad; i++) m &= m - 1ull;  // the quad-th active lane
+    bool qact = m != 0ull;
+    const int src = qact ? __ffsll((long long)m) - 1 : lane;
+    double y[2] = {0.0, 0.0}, k1[2] = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double vy = from_lane(y6[j], src), vk = from_lane(k16[j], src);
+        if (j == axis) {
+            y[0] = vy;
+            k1[0] = vk;
+        }
+        if (j == axis + 3) {
+            y[1] = vy;
+            k1[1] = vk;
+        }
+    }
+    DopState s;
+    s.orb = from_lane(s6.orb, src);
+    s.x = from_lane(s6.x, src);
+    s.xend = from_lane(s6.xend, src);
+    s.h = from_lane(s6.h, src);
+    s.hmax = from_lane(s6.hmax, src);
+    s.posneg = from_lane(s6.posneg, src);
+    s.facold = from_lane(s6.facold, src);
+    s.hlamb = from_lane(s6.hlamb, src);
+    s.nstep = from_lane(s6.nstep, src);
+    s.naccpt = from_lane(s6.naccpt, src);
+    s.nrejct = from_lane(s6.nrejct, src);
+    s.nfcn = from_lane(s6.nfcn, src);
+    s.oidx = from_lane(s6.oidx, src);
+    s.reject = from_lane(s6.reject, src);
+    s.iasti = from_lane(s6.iasti, src);
+    s.nonsti = from_lane(s6.nonsti, src);
+    const QuadPair<SCF, LAY> qsys{axis};
+    // every quad ends: each attempt advances nstep, and nmax bounds it
+    while (__popcll(__ballot(qact && c == 0)) > WIDE_ORBITS) {
+        if (qact) {
+            const int code = dop_attempt<SCF, false, LAY>(P, a, qsys, s, y, k1);
+            if (code != 0) {
+                if (c < 3) {
+                    a.w_final[axis * a.N + s.orb] = y[0];
+                    a.w_final[(axis + 3) * a.N + s.orb] = y[1];
+                }
+                if (c == 0) finish_stats(a, s, code);
+                qact = false;
+            }
+        }
+    }
+    // the remaining orbits back in one lane each (lane 0 of their quad), then rows
+    double yb[6], kb[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const int from = 4 * quad + (j % 3);
+        yb[j] = from_lane(j < 3 ? y[0] : y[1], from);
+        kb[j] = from_lane(j < 3 ? k1[0] : k1[1], from);
+    }
+    wide_tail<SCF, LAY>(P, a, qact && c == 0, s, yb, kb);
+}
+
 // start the orbit at queue slot `slot`: state, first derivative and, for
 // h0 = 0, Hairer's initial step (dopcor's set-up, dop853.cpp:300-320)
 template <bool SCF, int LAY>
@@ -695,8 +799,13 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         if (!DENSE && a.wide) {
             // every idle lane has found the queue empty; with few orbits
             // left, spread them over rows and finish them there
-            if (__popcll(__ballot(active)) <= WIDE_ORBITS) {
+            const int left = __popcll(__ballot(active));
+            if (left <= WIDE_ORBITS) {
                 wide_tail<SCF, LAY>(P, a, active, s, y, k1);
+                break;
+            }
+            if (left <= QUAD_ORBITS && a.wide > 1) {
+                quad_tail<SCF, LAY>(P, a, active, s, y, k1);
                 break;
             }
         }
@@ -897,9 +1006,10 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     a.order = order;
     a.n_tail = n_tail;
     a.queue = queue;
-    // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end (same results)
+    // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end, =1 skips the
+    // quad stage (same results every way)
     const char *wide_env = getenv("GAMD_DOP853_WIDE");
-    a.wide = !(wide_env && wide_env[0] == '0');
+    a.wide = !wide_env ? 2 : (wide_env[0] == '0' ? 0 : (wide_env[0] == '1' ? 1 : 2));
     a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
     int rc;
     const bool scf = pot->dev.has_scf;

@@ -108,10 +108,12 @@ def test_queue_order_invariance():
 
 @pytest.mark.parametrize("layout", ["milky_way", "generic"])
 def test_wide_tail_matches_one_lane_per_orbit(layout, monkeypatch):
-    """The last orbits of each wave finish spread over 16-lane rows; that
-    changes the lane layout only, never a bit of the result (final states,
-    status codes and step statistics), including the stiffness test, which
-    nstiff=3 runs every few steps."""
+    """The last orbits of each wave finish spread over 4-lane quads (at most
+    16 left) and then 16-lane rows (at most 4 left); that changes the lane
+    layout only, never a bit of the result (final states, status codes and
+    step statistics), including the stiffness test, which nstiff=3 runs every
+    few steps.  GAMD_DOP853_WIDE: 0 one lane to the end, 1 rows only, 2 (the
+    default) quads then rows."""
     from gala_amd.integrate import dop853
     from gala_amd import _device as dev
     from gala_amd.potential import HernquistPotential, PlummerPotential
@@ -124,15 +126,16 @@ def test_wide_tail_matches_one_lane_per_orbit(layout, monkeypatch):
         pot = HernquistPotential(m=1e11, c=2.0) + PlummerPotential(m=1e10, b=1.0,
                                                                    origin=[1.0, 0.5, -0.2])
     out = {}
-    for wide in ("0", "1"):
+    for wide in ("0", "1", "2"):
         monkeypatch.setenv("GAMD_DOP853_WIDE", wide)
         for nstiff in (0, 3):
             r = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, nstiff=nstiff)
             out[wide, nstiff] = [dev.to_host(r[k]) for k in ("w_final", "status", "stats")]
-    for nstiff in (0, 3):
-        for a, b in zip(out["0", nstiff], out["1", nstiff]):
-            assert np.array_equal(a, b)
-    assert np.all(out["1", 0][1] == 1)
+    for wide in ("1", "2"):
+        for nstiff in (0, 3):
+            for a, b in zip(out["0", nstiff], out[wide, nstiff]):
+                assert np.array_equal(a, b)
+    assert np.all(out["2", 0][1] == 1)
 
 
 def test_dedicated_tail_waves_match():
