@@ -89,7 +89,7 @@ def bench_dop853(args):
         tuned = autotune.tune_dop853(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, order=ordd,
                                      blocks=autotune.BLOCKS if block == 0 else (block,),
                                      tails=autotune.TAILS if n_tail is None else (n_tail,),
-                                     repeats=2, log=log)
+                                     repeats=3, log=log)
         block, n_tail = tuned["block"], tuned["n_tail"]
         log(f"autotuned: block {block}, n_tail {n_tail}")
 

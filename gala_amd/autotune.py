@@ -10,7 +10,6 @@ potential, tolerance and problem size, in memory and in a JSON file
 set it to an empty string to keep results in memory only).
 """
 import json
-import math
 import os
 
 from . import _device as dev
@@ -73,9 +72,12 @@ def lookup_dop853(potential, N, rtol, atol, dense=False):
 
 
 def _time_ms(fn, repeats):
+    """median of `repeats` timed calls (pass times of the persistent kernel
+    scatter by several percent with the dynamic queue; the median rewards a
+    shape that is fast every time)"""
     t = dev.torch()
     s = t.cuda.current_stream()
-    best = math.inf
+    ts = []
     for _ in range(repeats):
         e0 = t.cuda.Event(enable_timing=True)
         e1 = t.cuda.Event(enable_timing=True)
@@ -83,8 +85,9 @@ def _time_ms(fn, repeats):
         fn()
         e1.record(s)
         e1.synchronize()
-        best = min(best, e0.elapsed_time(e1))
-    return best
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    return ts[len(ts) // 2]
 
 
 def tune_dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, order=None,
