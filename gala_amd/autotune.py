@@ -72,9 +72,9 @@ def lookup_dop853(potential, N, rtol, atol, dense=False):
 
 
 def _time_ms(fn, repeats):
-    """median of `repeats` timed calls (pass times of the persistent kernel
-    scatter by several percent with the dynamic queue; the median rewards a
-    shape that is fast every time)"""
+    """mean of `repeats` timed calls (pass times of the persistent kernel
+    scatter with the dynamic queue, and some shapes now and then run 30 %
+    slower; the mean rewards a shape that is fast every time)"""
     t = dev.torch()
     s = t.cuda.current_stream()
     ts = []
@@ -86,8 +86,7 @@ def _time_ms(fn, repeats):
         e1.record(s)
         e1.synchronize()
         ts.append(e0.elapsed_time(e1))
-    ts.sort()
-    return ts[len(ts) // 2]
+    return sum(ts) / len(ts)
 
 
 def tune_dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, order=None,
