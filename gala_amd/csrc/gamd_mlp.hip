@@ -150,28 +150,38 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
                 for (int mi = 0; mi < 8; mi++) {
                     const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
                         in + (16 * mi + (lane & 15)) * LDSR + 32 * ks + 8 * (lane >> 4));
+                    // the transposed product, Z^T = W^T H^T: the weight fragment
+                    // is the A operand and the activations the B operand, so a
+                    // lane's four accumulators are four consecutive units of
+                    // one point (one 8-byte store below instead of four 2-byte)
 #pragma unroll
                     for (int ni = 0; ni < 2; ni++)
                         acc[mi][ni] =
-                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ni], acc[mi][ni], 0, 0, 0);
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[ni], a, acc[mi][ni], 0, 0, 0);
                 }
             }
 #pragma unroll
             for (int mi = 0; mi < 8; mi++)
 #pragma unroll
                 for (int ni = 0; ni < 2; ni++) {
-                    const int col = 64 * w + 16 * (2 * half + ni) + (lane & 15);
-                    const float bc = bias[col];
+                    const int unit = 64 * w + 16 * (2 * half + ni) + 4 * (lane >> 4);
+                    const int row = 16 * mi + (lane & 15);
+                    const f32x4 bc = *reinterpret_cast<const f32x4 *>(bias + unit);
+                    __bf16 hv[4], zv[4];
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        const int row = 16 * mi + 4 * (lane >> 4) + e;
-                        const float z = acc[mi][ni][e] + bc;
-                        const float h = silu(z);
-                        outb[row * LDSR + col] = tobf(h);
-                        if (TRAIN && row0 + row < M) {
-                            Zs[((size_t)(l + 1) * M + row0 + row) * H + col] = tobf(z);
-                            Hs[((size_t)(l + 1) * M + row0 + row) * H + col] = tobf(h);
-                        }
+                        const float z = acc[mi][ni][e] + bc[e];
+                        hv[e] = tobf(silu(z));
+                        zv[e] = tobf(z);
+                    }
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    *reinterpret_cast<bf16x4 *>(outb + row * LDSR + unit) =
+                        bf16x4{hv[0], hv[1], hv[2], hv[3]};
+                    if (TRAIN && row0 + row < M) {
+                        *reinterpret_cast<bf16x4 *>(Zs + ((size_t)(l + 1) * M + row0 + row) * H + unit) =
+                            bf16x4{zv[0], zv[1], zv[2], zv[3]};
+                        *reinterpret_cast<bf16x4 *>(Hs + ((size_t)(l + 1) * M + row0 + row) * H + unit) =
+                            bf16x4{hv[0], hv[1], hv[2], hv[3]};
                     }
                 }
         }
