@@ -27,7 +27,9 @@
 // four stay in registers, more spill to scratch).  The bodies' arithmetic is
 // identical in every lane, so each lane reproduces the reference's bodies
 // exactly, and the uneven per-release lengths (0 .. nfull-1 steps) become
-// independent lanes of one launch.
+// independent lanes of one launch.  The external potential is evaluated by
+// the straight-line Milky-Way form when its table has that layout (the DOP853
+// N-body kernels choose the same way), else by the general component loop.
 #include <math.h>
 #include <string.h>
 
@@ -41,11 +43,11 @@ namespace {
 // external potential, then every other body with a potential, centred on that
 // body's position, in body order (c_gradient, then c_nbody_gradient_symplectic
 // adding each body's f2).
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void nbody_grad(const DevPot &E, const BodyTable &bt, int self,
                                            const double (&bx)[KMAX][3], double x, double y,
                                            double z, double &gx, double &gy, double &gz) {
-    pot_grad<SCF>(E, x, y, z, gx, gy, gz);
+    pot_grad_l<SCF, LAY>(E, x, y, z, gx, gy, gz);
 #pragma unroll
     for (int j = 0; j < KMAX; j++) {
         if (j < bt.n && j != self && bt.p[j]) {
@@ -67,14 +69,14 @@ struct Bodies {
 
 // c_init_velocity_nbody for bodies 0..K-1 (positions unchanged, so the order
 // does not matter here)
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void bodies_init(const DevPot &E, const BodyTable &bt, int K,
                                             double dt, Bodies<KMAX> &B) {
 #pragma unroll
     for (int i = 0; i < KMAX; i++) {
         if (i < K) {
             double gx, gy, gz;
-            nbody_grad<SCF, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
+            nbody_grad<SCF, LAY, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
             B.vh[i][0] = B.v[i][0] - gx * dt / 2.0;
             B.vh[i][1] = B.v[i][1] - gy * dt / 2.0;
             B.vh[i][2] = B.v[i][2] - gz * dt / 2.0;
@@ -83,7 +85,7 @@ __device__ __forceinline__ void bodies_init(const DevPot &E, const BodyTable &bt
 }
 
 // one c_leapfrog_step_nbody for bodies 0..K-1, in index order
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void bodies_step(const DevPot &E, const BodyTable &bt, int K,
                                             double dt, Bodies<KMAX> &B) {
 #pragma unroll
@@ -93,7 +95,7 @@ __device__ __forceinline__ void bodies_step(const DevPot &E, const BodyTable &bt
             B.x[i][1] = B.x[i][1] + B.vh[i][1] * dt;
             B.x[i][2] = B.x[i][2] + B.vh[i][2] * dt;
             double gx, gy, gz;
-            nbody_grad<SCF, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
+            nbody_grad<SCF, LAY, KMAX>(E, bt, i, B.x, B.x[i][0], B.x[i][1], B.x[i][2], gx, gy, gz);
             B.v[i][0] = B.vh[i][0] - gx * dt / 2.0;
             B.v[i][1] = B.vh[i][1] - gy * dt / 2.0;
             B.v[i][2] = B.vh[i][2] - gz * dt / 2.0;
@@ -109,24 +111,24 @@ struct Particle {
     double x, y, z, vx, vy, vz, hx, hy, hz;
 };
 
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void particle_init(const DevPot &E, const BodyTable &bt,
                                               const Bodies<KMAX> &B, double dt, Particle &p) {
     double gx, gy, gz;
-    nbody_grad<SCF, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
+    nbody_grad<SCF, LAY, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
     p.hx = p.vx - gx * dt / 2.0;
     p.hy = p.vy - gy * dt / 2.0;
     p.hz = p.vz - gz * dt / 2.0;
 }
 
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void particle_step(const DevPot &E, const BodyTable &bt,
                                               const Bodies<KMAX> &B, double dt, Particle &p) {
     p.x = p.x + p.hx * dt;
     p.y = p.y + p.hy * dt;
     p.z = p.z + p.hz * dt;
     double gx, gy, gz;
-    nbody_grad<SCF, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
+    nbody_grad<SCF, LAY, KMAX>(E, bt, -1, B.x, p.x, p.y, p.z, gx, gy, gz);
     p.vx = p.hx - gx * dt / 2.0;
     p.vy = p.hy - gy * dt / 2.0;
     p.vz = p.hz - gz * dt / 2.0;
@@ -137,7 +139,7 @@ __device__ __forceinline__ void particle_step(const DevPot &E, const BodyTable &
 
 // leapfrog_integrate_nbody: lane l integrates the K massive bodies and test
 // particle K + l (when there is one); lane 0 writes the bodies.
-template <bool SCF, bool SAVE, int KMAX>
+template <bool SCF, int LAY, bool SAVE, int KMAX>
 __global__ __launch_bounds__(64) void k_nbody_leapfrog(const DevPot *__restrict__ Ep, BodyTable bt,
                                                        const double *__restrict__ w0, int64_t n,
                                                        double dt, int64_t n_steps, int64_t stride,
@@ -194,11 +196,11 @@ __global__ __launch_bounds__(64) void k_nbody_leapfrog(const DevPot *__restrict_
         }
     };
     if (SAVE) store(wout, nsave, 0);
-    bodies_init<SCF, KMAX>(E, bt, K, dt, B);
-    if (has_p) particle_init<SCF, KMAX>(E, bt, B, dt, p);
+    bodies_init<SCF, LAY, KMAX>(E, bt, K, dt, B);
+    if (has_p) particle_init<SCF, LAY, KMAX>(E, bt, B, dt, p);
     for (int64_t s = 1; s <= n_steps; s++) {
-        bodies_step<SCF, KMAX>(E, bt, K, dt, B);
-        if (has_p) particle_step<SCF, KMAX>(E, bt, B, dt, p);
+        bodies_step<SCF, LAY, KMAX>(E, bt, K, dt, B);
+        if (has_p) particle_step<SCF, LAY, KMAX>(E, bt, B, dt, p);
         if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
     }
     if (wfinal) store(wfinal, 1, 0);
@@ -213,7 +215,7 @@ struct Ruth4Coef {
 // against the bodies' current positions: four kick-then-drift substeps.  The
 // first kick has d_0 = 0 and adds exactly zero wherever the force is finite,
 // so its force evaluation is skipped.
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __device__ __forceinline__ void ruth4_substeps(const DevPot &E, const BodyTable &bt, int self,
                                                const double (&bx)[KMAX][3], double dt,
                                                const Ruth4Coef &cf, double (&x)[3],
@@ -223,7 +225,7 @@ __device__ __forceinline__ void ruth4_substeps(const DevPot &E, const BodyTable 
 #pragma unroll
     for (int k = 1; k < 4; k++) {
         double g[3];
-        nbody_grad<SCF, KMAX>(E, bt, self, bx, x[0], x[1], x[2], g[0], g[1], g[2]);
+        nbody_grad<SCF, LAY, KMAX>(E, bt, self, bx, x[0], x[1], x[2], g[0], g[1], g[2]);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             v[c] = v[c] - cf.d[k] * g[c] * dt;
@@ -236,7 +238,7 @@ __device__ __forceinline__ void ruth4_substeps(const DevPot &E, const BodyTable 
 // order, all four substeps of one body before the next (bodies below i have
 // already moved a full step when body i moves); lane l carries the bodies
 // and test particle K + l.  Lane 0 writes the bodies.
-template <bool SCF, bool SAVE, int KMAX>
+template <bool SCF, int LAY, bool SAVE, int KMAX>
 __global__ __launch_bounds__(64) void k_nbody_ruth4(const DevPot *__restrict__ Ep, BodyTable bt,
                                                     const double *__restrict__ w0, int64_t n,
                                                     double dt, int64_t n_steps, int64_t stride,
@@ -293,8 +295,8 @@ __global__ __launch_bounds__(64) void k_nbody_ruth4(const DevPot *__restrict__ E
     for (int64_t s = 1; s <= n_steps; s++) {
 #pragma unroll
         for (int i = 0; i < KMAX; i++)
-            if (i < K) ruth4_substeps<SCF, KMAX>(E, bt, i, bx, dt, cf, bx[i], bv[i]);
-        if (has_p) ruth4_substeps<SCF, KMAX>(E, bt, -1, bx, dt, cf, px, pv);
+            if (i < K) ruth4_substeps<SCF, LAY, KMAX>(E, bt, i, bx, dt, cf, bx[i], bv[i]);
+        if (has_p) ruth4_substeps<SCF, LAY, KMAX>(E, bt, -1, bx, dt, cf, px, pv);
         if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
     }
     if (wfinal) store(wfinal, 1, 0);
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(64) void k_nbody_ruth4(const DevPot *__restrict__ E
 
 // mockstream_leapfrog's release batches: lane i = particle i with its own
 // copy of the bodies from their states at its release.
-template <bool SCF, int KMAX>
+template <bool SCF, int LAY, int KMAX>
 __global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep, BodyTable bt,
                                                    const double *__restrict__ traj, int64_t nfull,
                                                    double t0, double dt,
@@ -336,11 +338,11 @@ __global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep
     p.vy = w0[4 * n + i];
     p.vz = w0[5 * n + i];
     // c_init_velocity_nbody: the bodies, then the particle
-    bodies_init<SCF, KMAX>(E, bt, K, dt, B);
-    particle_init<SCF, KMAX>(E, bt, B, dt, p);
+    bodies_init<SCF, LAY, KMAX>(E, bt, K, dt, B);
+    particle_init<SCF, LAY, KMAX>(E, bt, B, dt, p);
     for (int64_t s = 0; s < nsteps; s++) {
-        bodies_step<SCF, KMAX>(E, bt, K, dt, B);
-        particle_step<SCF, KMAX>(E, bt, B, dt, p);
+        bodies_step<SCF, LAY, KMAX>(E, bt, K, dt, B);
+        particle_step<SCF, LAY, KMAX>(E, bt, B, dt, p);
     }
     out[i] = p.x;
     out[n + i] = p.y;
@@ -348,6 +350,53 @@ __global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep
     out[3 * n + i] = p.vx;
     out[4 * n + i] = p.vy;
     out[5 * n + i] = p.vz;
+}
+
+// The external potential's evaluator: the straight-line Milky-Way form when
+// the table has that layout and no body is an SCF expansion (as the DOP853
+// N-body kernels choose), else the general component loop, with the SCF
+// branch compiled in only when some potential needs it.  Bodies per lane:
+// 1, up to 4 (registers), up to 8 (the body arrays spill to scratch).
+enum { EV_GENERIC, EV_SCF, EV_MW };
+
+int evaluator(const gamd_potential *ext, const BodyTable &bt) {
+    if (ext->dev.has_scf || bt.scf) return EV_SCF;
+    return ext->dev.layout == LAYOUT_MW ? EV_MW : EV_GENERIC;
+}
+
+template <bool SCF, int LAY, bool SAVE>
+auto leapfrog_kernel(int nbody) {
+    return nbody <= 1   ? k_nbody_leapfrog<SCF, LAY, SAVE, 1>
+           : nbody <= 4 ? k_nbody_leapfrog<SCF, LAY, SAVE, 4>
+                        : k_nbody_leapfrog<SCF, LAY, SAVE, 8>;
+}
+
+template <bool SAVE>
+auto leapfrog_kernel(int ev, int nbody) {
+    return ev == EV_SCF  ? leapfrog_kernel<true, LAYOUT_GENERIC, SAVE>(nbody)
+           : ev == EV_MW ? leapfrog_kernel<false, LAYOUT_MW, SAVE>(nbody)
+                         : leapfrog_kernel<false, LAYOUT_GENERIC, SAVE>(nbody);
+}
+
+template <bool SCF, int LAY, bool SAVE>
+auto ruth4_kernel(int nbody) {
+    return nbody <= 1   ? k_nbody_ruth4<SCF, LAY, SAVE, 1>
+           : nbody <= 4 ? k_nbody_ruth4<SCF, LAY, SAVE, 4>
+                        : k_nbody_ruth4<SCF, LAY, SAVE, 8>;
+}
+
+template <bool SAVE>
+auto ruth4_kernel(int ev, int nbody) {
+    return ev == EV_SCF  ? ruth4_kernel<true, LAYOUT_GENERIC, SAVE>(nbody)
+           : ev == EV_MW ? ruth4_kernel<false, LAYOUT_MW, SAVE>(nbody)
+                         : ruth4_kernel<false, LAYOUT_GENERIC, SAVE>(nbody);
+}
+
+template <bool SCF, int LAY>
+auto mockstream_kernel(int nbody) {
+    return nbody <= 1   ? k_mockstream<SCF, LAY, 1>
+           : nbody <= 4 ? k_mockstream<SCF, LAY, 4>
+                        : k_mockstream<SCF, LAY, 8>;
 }
 
 }  // namespace
@@ -366,17 +415,8 @@ extern "C" int gamd_nbody_leapfrog(const gamd_potential *ext, const gamd_potenti
     const int64_t nsave = w_out ? n_steps / save_stride + 1 : 0;
     const int64_t lanes = (int64_t)n > nbody ? (int64_t)n - nbody : 1;
     const unsigned block = 64;
-    const bool scf = ext->dev.has_scf || bt.scf;
-    // bodies per lane: 1, up to 4 (registers), up to 8 (the body arrays spill to scratch)
-    auto pick = [&](auto k1, auto k4, auto k8) { return nbody <= 1 ? k1 : nbody <= 4 ? k4 : k8; };
-    auto kern = w_out ? (scf ? pick(k_nbody_leapfrog<true, true, 1>, k_nbody_leapfrog<true, true, 4>,
-                                    k_nbody_leapfrog<true, true, 8>)
-                             : pick(k_nbody_leapfrog<false, true, 1>, k_nbody_leapfrog<false, true, 4>,
-                                    k_nbody_leapfrog<false, true, 8>))
-                      : (scf ? pick(k_nbody_leapfrog<true, false, 1>, k_nbody_leapfrog<true, false, 4>,
-                                    k_nbody_leapfrog<true, false, 8>)
-                             : pick(k_nbody_leapfrog<false, false, 1>, k_nbody_leapfrog<false, false, 4>,
-                                    k_nbody_leapfrog<false, false, 8>));
+    const int ev = evaluator(ext, bt);
+    auto kern = w_out ? leapfrog_kernel<true>(ev, nbody) : leapfrog_kernel<false>(ev, nbody);
     hipLaunchKernelGGL(kern, dim3(grid_for((size_t)lanes, block)), dim3(block), 0, as_stream(stream),
                        ext->ddev, bt, w0, (int64_t)n, dt, n_steps, w_out ? save_stride : 1, nsave,
                        w_out, w_final);
@@ -408,16 +448,8 @@ extern "C" int gamd_nbody_ruth4(const gamd_potential *ext, const gamd_potential 
     const int64_t nsave = w_out ? n_steps / save_stride + 1 : 0;
     const int64_t lanes = (int64_t)n > nbody ? (int64_t)n - nbody : 1;
     const unsigned block = 64;
-    const bool scf = ext->dev.has_scf || bt.scf;
-    auto pick = [&](auto k1, auto k4, auto k8) { return nbody <= 1 ? k1 : nbody <= 4 ? k4 : k8; };
-    auto kern = w_out ? (scf ? pick(k_nbody_ruth4<true, true, 1>, k_nbody_ruth4<true, true, 4>,
-                                    k_nbody_ruth4<true, true, 8>)
-                             : pick(k_nbody_ruth4<false, true, 1>, k_nbody_ruth4<false, true, 4>,
-                                    k_nbody_ruth4<false, true, 8>))
-                      : (scf ? pick(k_nbody_ruth4<true, false, 1>, k_nbody_ruth4<true, false, 4>,
-                                    k_nbody_ruth4<true, false, 8>)
-                             : pick(k_nbody_ruth4<false, false, 1>, k_nbody_ruth4<false, false, 4>,
-                                    k_nbody_ruth4<false, false, 8>));
+    const int ev = evaluator(ext, bt);
+    auto kern = w_out ? ruth4_kernel<true>(ev, nbody) : ruth4_kernel<false>(ev, nbody);
     hipLaunchKernelGGL(kern, dim3(grid_for((size_t)lanes, block)), dim3(block), 0, as_stream(stream),
                        ext->ddev, bt, w0, (int64_t)n, dt, n_steps, w_out ? save_stride : 1, nsave,
                        cf, w_out, w_final);
@@ -434,13 +466,10 @@ extern "C" int gamd_mockstream_leapfrog(const gamd_potential *ext,
     if (!ext || nbody < 1 || !body_traj || nfull == 0 || !w0 || !rel_t || !out || dt == 0.0)
         return GAMD_ERR_INVALID;
     if (n == 0) return GAMD_OK;
-    const bool scf = ext->dev.has_scf || bt.scf;
-    auto kern = scf ? (nbody == 1   ? k_mockstream<true, 1>
-                       : nbody <= 4 ? k_mockstream<true, 4>
-                                    : k_mockstream<true, 8>)
-                    : (nbody == 1   ? k_mockstream<false, 1>
-                       : nbody <= 4 ? k_mockstream<false, 4>
-                                    : k_mockstream<false, 8>);
+    const int ev = evaluator(ext, bt);
+    auto kern = ev == EV_SCF  ? mockstream_kernel<true, LAYOUT_GENERIC>(nbody)
+                : ev == EV_MW ? mockstream_kernel<false, LAYOUT_MW>(nbody)
+                              : mockstream_kernel<false, LAYOUT_GENERIC>(nbody);
     const unsigned block = 64;
     hipLaunchKernelGGL(kern, dim3(grid_for(n, block)), dim3(block), 0, as_stream(stream),
                        ext->ddev, bt, body_traj, (int64_t)nfull, t0, dt, w0, rel_t, (int64_t)n,

@@ -207,15 +207,13 @@ class BaseStreamDF:
             n_particles = np.zeros_like(prog_t, dtype="i4")
             n_particles[::release_every] = N
         x, v, t1 = self._sample(H.potential, prog_x, prog_v, prog_t, prog_m, n_particles)
-        lt = np.empty(len(t1), dtype="U1")
-        i = 0
-        for n in n_particles:
-            if self._trail:
-                lt[i:i + n] = "t"
-                i += n
-            if self._lead:
-                lt[i:i + n] = "l"
-                i += n
+        # per release time a trailing block then a leading block (the
+        # reference's loop over n_particles, whose slices past the end of the
+        # sampled particles are empty)
+        labels = np.array([c for c, on in (("t", self._trail), ("l", self._lead)) if on],
+                          dtype="U1")
+        lt = np.repeat(np.tile(labels, len(n_particles)),
+                       np.repeat(np.asarray(n_particles, dtype=np.int64), len(labels)))[:len(t1)]
         return MockStream(pos=x.T, vel=v.T, release_time=t1, lead_trail=lt,
                           frame=StaticFrame(H.units))
 
@@ -248,7 +246,7 @@ class FardalStreamDF(BaseStreamDF):
         counts = np.asarray(nparticles)[idx]
         # particle order: per release time, trailing block then leading block
         rel = np.repeat(np.arange(len(idx)), counts * len(tails))
-        sign = np.concatenate([np.repeat(np.array(tails), c) for c in counts]) if len(idx) else np.zeros(0)
+        sign = np.repeat(np.tile(np.array(tails), len(counts)), np.repeat(counts, len(tails)))
         n = len(rel)
         z = self._draws(4 * n).reshape(n, 4)
         kx = 2.0 + kd0 * z[:, 0]
@@ -276,7 +274,7 @@ class StreaklineStreamDF(BaseStreamDF):
                                   prog_t[idx])
         counts = np.asarray(nparticles)[idx]
         rel = np.repeat(np.arange(len(idx)), counts * len(tails))
-        sign = np.concatenate([np.repeat(np.array(tails), c) for c in counts]) if len(idx) else np.zeros(0)
+        sign = np.repeat(np.tile(np.array(tails), len(counts)), np.repeat(counts, len(tails)))
         n = len(rel)
         tx = np.zeros((n, 3))
         tv = np.zeros((n, 3))
