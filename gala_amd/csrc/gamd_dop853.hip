@@ -270,6 +270,14 @@ struct QuadPair {
 // are left out of every sum.
 //   QuadBodies: one system of at most 4 bodies per 4-lane quad (several
 //               systems per wave), DPP quad_perm broadcasts;
+//   RowBodies:  one system of at most 4 bodies per 16-lane row, body b on
+//               the row's quad b and lane c < 3 of the quad holding the
+//               position and velocity along axis c (lane 3 mirrors lane 0):
+//               the stage combinations cost two components' arithmetic
+//               instead of six.  Body positions are broadcast with DPP
+//               quad_perm (own body) and row_newbcast (other bodies); sums
+//               follow the body-major order, so results are bit-identical to
+//               QuadBodies;
 //   WaveBodies: one system of at most 64 bodies per wave, readlane
 //               broadcasts (wave-uniform source lane).
 template <bool SCF, int LAY>
@@ -322,6 +330,64 @@ struct QuadBodies {
         }
     }
     __device__ __forceinline__ void sums(const double (&a)[6], const double (&b)[6], double &sa,
+                                         double &sb) const {
+        sums_body<0>(a, b, sa, sb);
+        sums_body<1>(a, b, sa, sb);
+        sums_body<2>(a, b, sa, sb);
+        sums_body<3>(a, b, sa, sb);
+    }
+    __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
+};
+
+// the RowBodies layout of the list above
+template <bool SCF, int LAY>
+struct RowBodies {
+    static constexpr int NC = 2;
+    int body, axis, nb;   // axis 0..2 (lane 3 of a quad runs axis 0 again)
+    const BodyTable &bt;  // bt.n <= nb <= 4
+    template <int J>
+    __device__ __forceinline__ void add_body(const double (&y)[2], double qx, double qy, double qz,
+                                             double &gx, double &gy, double &gz) const {
+        if (J < bt.n) {  // wave-uniform
+            const double px = row_bcast<4 * J>(y[0]), py = row_bcast<4 * J + 1>(y[0]),
+                         pz = row_bcast<4 * J + 2>(y[0]);
+            if (J != body && body < nb && bt.p[J]) {
+                double fx, fy, fz;
+                body_grad<SCF>(*bt.p[J], qx, qy, qz, px, py, pz, fx, fy, fz);
+                gx += fx;
+                gy += fy;
+                gz += fz;
+            }
+        }
+    }
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[2], double (&f)[2]) const {
+        const double qx = quad_bcast<0>(y[0]), qy = quad_bcast<1>(y[0]), qz = quad_bcast<2>(y[0]);
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(P, qx, qy, qz, gx, gy, gz);
+        add_body<0>(y, qx, qy, qz, gx, gy, gz);
+        add_body<1>(y, qx, qy, qz, gx, gy, gz);
+        add_body<2>(y, qx, qy, qz, gx, gy, gz);
+        add_body<3>(y, qx, qy, qz, gx, gy, gz);
+        f[0] = y[1];
+        f[1] = -(axis == 0 ? gx : (axis == 1 ? gy : gz));
+    }
+    template <int B>
+    __device__ __forceinline__ void sums_body(const double (&a)[2], const double (&b)[2],
+                                              double &sa, double &sb) const {
+        const double qa[6] = {row_bcast<4 * B>(a[0]), row_bcast<4 * B + 1>(a[0]),
+                              row_bcast<4 * B + 2>(a[0]), row_bcast<4 * B>(a[1]),
+                              row_bcast<4 * B + 1>(a[1]), row_bcast<4 * B + 2>(a[1])};
+        const double qb[6] = {row_bcast<4 * B>(b[0]), row_bcast<4 * B + 1>(b[0]),
+                              row_bcast<4 * B + 2>(b[0]), row_bcast<4 * B>(b[1]),
+                              row_bcast<4 * B + 1>(b[1]), row_bcast<4 * B + 2>(b[1])};
+        if (B < nb) {
+            FOR6 {
+                sa += qa[c] * qa[c];
+                sb += qb[c] * qb[c];
+            }
+        }
+    }
+    __device__ __forceinline__ void sums(const double (&a)[2], const double (&b)[2], double &sa,
                                          double &sb) const {
         sums_body<0>(a, b, sa, sb);
         sums_body<1>(a, b, sa, sb);
@@ -842,38 +908,58 @@ struct StreamArgs {
     int64_t *stats;           // (nrel, 4) or nullptr
 };
 
-// G = 4: QuadBodies; G = 64: WaveBodies.  quads: the quad launch takes the
-// systems of at most 4 bodies (so the wave launch skips them).
+// G = 4: QuadBodies; G = 16: RowBodies; G = 64: WaveBodies.  quads: the
+// quad or row launch takes the systems of at most 4 bodies (so the wave
+// launch skips them).
 template <bool SCF, int LAY, int G>
 __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict__ Pp, BodyTable bt,
                                                        int quads, StreamArgs sa, DopArgs a) {
     const DevPot &P = *Pp;
     const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t rel = gid / G;  // uniform over the group
-    const int body = (int)(gid % G);
+    const int body = G == 16 ? (int)(gid % 16) / 4 : (int)(gid % G);
+    const int lane4 = (int)(gid % 4);
+    const int axis = lane4 < 3 ? lane4 : 0;
     if (rel >= sa.nrel) return;
     const int K = bt.n;
     const int k = sa.nstream[rel];
+    const bool lead = G == 16 ? (gid % 16) == 0 : body == 0;
     if (k <= 0) {
-        if (body == 0 && (G == 4 || !quads)) sa.status[rel] = 0;
+        if (lead && (G != 64 || !quads)) sa.status[rel] = 0;
         return;
     }
     const int nb = K + k;
-    if (G == 4 && nb > 4) return;              // a wave's system
-    if (G == 64 && quads && nb <= 4) return;   // a quad's system
-    if (nb > G) {
-        if (body == 0) sa.status[rel] = GAMD_ERR_UNSUPPORTED;
+    if (G != 64 && nb > 4) return;             // a wave's system
+    if (G == 64 && quads && nb <= 4) return;   // a quad's or row's system
+    if (nb > (G == 16 ? 4 : G)) {
+        if (lead) sa.status[rel] = GAMD_ERR_UNSUPPORTED;
         return;
     }
-    using Sys = typename std::conditional<G == 4, QuadBodies<SCF, LAY>, WaveBodies<SCF, LAY>>::type;
-    const Sys sys{body, nb, bt};
-    double y[6], k1[6];
+    using Sys = typename std::conditional<
+        G == 4, QuadBodies<SCF, LAY>,
+        typename std::conditional<G == 16, RowBodies<SCF, LAY>, WaveBodies<SCF, LAY>>::type>::type;
+    constexpr int NC = Sys::NC;
+    auto make_sys = [&]() {
+        if constexpr (G == 16)
+            return Sys{body, axis, nb, bt};
+        else
+            return Sys{body, nb, bt};
+    };
+    const Sys sys = make_sys();
+    double w[6];
     if (body >= K && body < nb) {
         const double *src = sa.stream_w0 + 6 * (sa.offset[rel] + body - K);
-        FOR6 y[c] = src[c];
+        FOR6 w[c] = src[c];
     } else {
         const int b = body < K ? body : 0;
-        FOR6 y[c] = sa.body_w[6 * (rel * K + b) + c];
+        FOR6 w[c] = sa.body_w[6 * (rel * K + b) + c];
+    }
+    double y[NC], k1[NC];
+    if constexpr (NC == 6) {
+        FOR6 y[c] = w[c];
+    } else {
+        y[0] = axis == 0 ? w[0] : (axis == 1 ? w[1] : w[2]);
+        y[1] = axis == 0 ? w[3] : (axis == 1 ? w[4] : w[5]);
     }
     DopState s;
     s.orb = rel;
@@ -892,12 +978,18 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
     int code = 0;
     // every group ends: each attempt advances nstep, and nmax bounds it
     while (code == 0) code = dop_attempt<SCF, false, LAY>(P, a, sys, s, y, k1);
-    if (body >= K && body < nb) {
-        double *dst = sa.out + 6 * (sa.offset[rel] + body - K);
-        FOR6 dst[c] = y[c];
+    double *dst = nullptr;
+    if (body >= K && body < nb) dst = sa.out + 6 * (sa.offset[rel] + body - K);
+    if (body < K) dst = sa.body_final + 6 * (rel * K + body);
+    if (dst) {
+        if constexpr (NC == 6) {
+            FOR6 dst[c] = y[c];
+        } else if (lane4 < 3) {
+            dst[axis] = y[0];
+            dst[axis + 3] = y[1];
+        }
     }
-    if (body < K) FOR6 sa.body_final[6 * (rel * K + body) + c] = y[c];
-    if (body == 0) {
+    if (lead) {
         sa.status[rel] = code;
         if (sa.stats) {
             sa.stats[4 * rel + 0] = s.nstep;
@@ -1069,16 +1161,24 @@ extern "C" int gamd_mockstream_dop853(const gamd_potential *ext,
     const int block = 256;
     hipStream_t s = as_stream(stream);
     if (quads) {
-        const unsigned grid = (unsigned)((4 * (int64_t)nrel + block - 1) / block);
-        if (scf)
-            hipLaunchKernelGGL((k_dop853_stream<true, LAYOUT_GENERIC, 4>), dim3(grid), dim3(block),
-                               0, s, ext->ddev, bt, quads, sa, a);
-        else if (mw)
-            hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_MW, 4>), dim3(grid), dim3(block), 0,
-                               s, ext->ddev, bt, quads, sa, a);
-        else
-            hipLaunchKernelGGL((k_dop853_stream<false, LAYOUT_GENERIC, 4>), dim3(grid),
-                               dim3(block), 0, s, ext->ddev, bt, quads, sa, a);
+        // Systems of at most 4 bodies: one per 16-lane row while the rows fit
+        // about two waves per SIMD (the launch is then bound by its longest
+        // systems, the earliest releases, whose attempts rows make shorter),
+        // else one per quad (a quarter of the lanes per system).
+        // GAMD_DOP853_STREAM_ROWS=0 / =1 forces quads / rows (same results).
+        const char *rows_env = getenv("GAMD_DOP853_STREAM_ROWS");
+        bool rows = 16 * (int64_t)nrel <= 2 * 64 * 4 * (int64_t)cu_count();
+        if (rows_env && rows_env[0] == '0') rows = false;
+        if (rows_env && rows_env[0] == '1') rows = true;
+        const int G = rows ? 16 : 4;
+        const unsigned grid = (unsigned)((G * (int64_t)nrel + block - 1) / block);
+        auto kern = rows ? (scf  ? k_dop853_stream<true, LAYOUT_GENERIC, 16>
+                            : mw ? k_dop853_stream<false, LAYOUT_MW, 16>
+                                 : k_dop853_stream<false, LAYOUT_GENERIC, 16>)
+                         : (scf  ? k_dop853_stream<true, LAYOUT_GENERIC, 4>
+                            : mw ? k_dop853_stream<false, LAYOUT_MW, 4>
+                                 : k_dop853_stream<false, LAYOUT_GENERIC, 4>);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, ext->ddev, bt, quads, sa, a);
         const int rc = launch_status();
         if (rc != GAMD_OK) return rc;
     }

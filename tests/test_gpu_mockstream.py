@@ -234,6 +234,59 @@ def test_dop853_release_kernel_matches_oracle():
     assert state_rel_err(dev.to_host(pf)[last][:, None], ref_pf[:, None])[0] < 1e-8
 
 
+@pytest.mark.parametrize("with_perturber", [False, True])
+def test_dop853_release_rows_match_quads(with_perturber, monkeypatch):
+    """Release systems of at most 4 bodies run one per 16-lane row (a body
+    per quad, an axis per lane) or one per 4-lane quad (a body per lane); the
+    two layouts must give bit-identical states, statuses and step counts.
+    GAMD_DOP853_STREAM_ROWS=0 forces quads, =1 rows."""
+    from gala_amd import _device as dev
+    from gala_amd import _lib
+    from gala_amd.potential import HernquistPotential, MilkyWayLikePotential, PlummerPotential
+    _, prog_mass, b, time, pw, ns, sw = _dop853_release_inputs(nrel=96, seed=7)
+    ext = MilkyWayLikePotential()
+    bodies = [PlummerPotential(m=prog_mass, b=b)]
+    body_w = pw[:, None, :]
+    if with_perturber:
+        # a 1e8 Msun clump 1.5 kpc from the progenitor, on a similar orbit
+        bodies.append(HernquistPotential(m=1e8, c=0.2))
+        pert = pw + np.array([1.5, 0.0, 0.0, 0.0, 0.01, 0.0])
+        body_w = np.stack([pw, pert], axis=1)
+    K = len(bodies)
+    body_w = np.ascontiguousarray(body_w)
+    t = dev.torch()
+    nrel, ntot = len(time), sw.shape[0]
+    off = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    keep = [dev.to_dev(body_w), dev.to_dev(time), dev.to_dev(ns, dtype=t.int32),
+            dev.to_dev(off, dtype=t.int64), dev.to_dev(sw)]
+    lib = _lib.load()
+    res = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("GAMD_DOP853_STREAM_ROWS", mode)
+        out = t.empty((ntot, 6), dtype=t.float64, device="cuda")
+        bf = t.empty((nrel, K, 6), dtype=t.float64, device="cuda")
+        status = t.empty(nrel, dtype=t.int32, device="cuda")
+        stats = t.empty((nrel, 4), dtype=t.int64, device="cuda")
+        _lib.check(lib.gamd_mockstream_dop853(ext.c_handle,
+                                              _lib.body_array([p.c_handle for p in bodies]), K,
+                                              keep[0].data_ptr(), nrel, keep[1].data_ptr(),
+                                              keep[2].data_ptr(), keep[3].data_ptr(),
+                                              keep[4].data_ptr(), 0.0, 5.0, 1e-10, 1e-10, 0,
+                                              out.data_ptr(), bf.data_ptr(), status.data_ptr(),
+                                              stats.data_ptr(), dev.stream_ptr()),
+                   "gamd_mockstream_dop853")
+        res[mode] = [dev.to_host(x) for x in (out, bf, status, stats)]
+    st = res["0"][2]
+    used = ns > 0  # a release without particles writes only its status
+    print(f"rows vs quads (K={K}): {np.sum(st == 1)} of {nrel} systems ok, "
+          f"{int(res['0'][3][used, 0].max())} attempts at most")
+    assert np.all(st[used] == 1) and np.all(st[~used] == 0)
+    assert np.array_equal(res["0"][0], res["1"][0])
+    assert np.array_equal(res["0"][2], res["1"][2])
+    for i in (1, 3):
+        assert np.array_equal(res["0"][i][used], res["1"][i][used])
+
+
 def test_dop853_generator_against_oracle_pipeline():
     """MockStreamGenerator.run with gala's default DOP853 integrator against
     the oracle pipeline fed the product's own DF samples: progenitor orbit by
