@@ -688,6 +688,10 @@ __device__ __forceinline__ void body_grad(const DevPot &PP, double x, double y, 
 __device__ __forceinline__ void mw_grad(const DevPot &P, double x, double y, double z,
                                         double &gx, double &gy, double &gz) {
     const DevComp &h1 = P.c[0], &h2 = P.c[1], &mn = P.c[2], &nf = P.c[3];
+    // parameters used inside the branches below, read before them: loads in
+    // a conditional block are not hoisted out of the caller's loop, and each
+    // one would stall the evaluation on the scalar cache
+    const double gm1 = h1.d[0], gm2 = h2.d[0], gmn = nf.d[0] * nf.d[1];
     const double R2 = x * x + y * y;
     const double r2 = R2 + z * z;
     // |q|, 1/|q|^3, sqrt(z^2 + b^2), its inverse and D^(-3/2) from three
@@ -729,16 +733,16 @@ __device__ __forceinline__ void mw_grad(const DevPot &P, double x, double y, dou
         const double inv3 = inv * tf;
         const double inv2 = inv3 * opu;
         i4 = inv3 * p12;
-        f1 = h1.d[0] * (inv2 * d2);
-        f2 = h2.d[0] * (inv2 * d1);
+        f1 = gm1 * (inv2 * d2);
+        f2 = gm2 * (inv2 * d1);
     } else {  // extreme radii (or r = 0): the per-component forms
         slog = lf / tf;
-        f1 = h1.d[0] / d1;
-        f2 = h2.d[0] / d2;
+        f1 = gm1 / d1;
+        f2 = gm2 / d2;
     }
     const double lg = log_series(slog, le);
     if (batched)
-        f4 = nf.d[0] * nf.d[1] * (lg - u * i4) * ir3;
+        f4 = gmn * (lg - u * i4) * ir3;
     else
         f4 = nf.d[0] / (u * u * u) / nf.d[2] * (lg - u / opu);
     gx = 0.0;

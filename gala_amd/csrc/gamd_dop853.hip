@@ -163,6 +163,7 @@ struct DopState {
     int64_t orb;
     double x, xend, h, hmax, posneg, facold, hlamb;
     int64_t nstep, naccpt, nrejct, nfcn, oidx;
+    int64_t stiff;  // naccpt % nstiff, kept incrementally (no 64-bit modulo per step)
     int reject, iasti, nonsti;
 };
 
@@ -531,10 +532,11 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
         // accepted (:454-637)
         s.facold = max_d(err, 1.0e-4);
         s.naccpt++;
+        if (++s.stiff == a.nstiff) s.stiff = 0;  // never 0 again when nstiff <= 0
         sys.rhs(P, k5, k4);  // FSAL
         s.nfcn++;
         // stiffness test (:463-486); returns -4 on the first detection
-        if (a.nstiff > 0 && (!(s.naccpt % a.nstiff) || s.iasti > 0)) {
+        if (a.nstiff > 0 && (s.stiff == 0 || s.iasti > 0)) {
             double dn[NC], dd[NC];
             FORN {
                 dn[c] = k4[c] - k3[c];
@@ -689,6 +691,7 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
     s.hlamb = from_lane(s6.hlamb, src);
     s.nstep = from_lane(s6.nstep, src);
     s.naccpt = from_lane(s6.naccpt, src);
+    s.stiff = from_lane(s6.stiff, src);
     s.nrejct = from_lane(s6.nrejct, src);
     s.nfcn = from_lane(s6.nfcn, src);
     s.oidx = from_lane(s6.oidx, src);
@@ -748,6 +751,7 @@ __device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, boo
     s.hlamb = from_lane(s6.hlamb, src);
     s.nstep = from_lane(s6.nstep, src);
     s.naccpt = from_lane(s6.naccpt, src);
+    s.stiff = from_lane(s6.stiff, src);
     s.nrejct = from_lane(s6.nrejct, src);
     s.nfcn = from_lane(s6.nfcn, src);
     s.oidx = from_lane(s6.oidx, src);
@@ -796,7 +800,7 @@ __device__ __forceinline__ void load_orbit(const DevPot &P, const DopArgs &a, in
     s.posneg = sign_d(1.0, s.xend - s.x);
     s.facold = 1.0e-4;
     s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
     s.reject = s.iasti = s.nonsti = 0;
@@ -816,7 +820,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
     s.posneg = 1.0;
     s.facold = 1e-4;
     s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.nrejct = s.nfcn = s.oidx = 0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = s.nfcn = s.oidx = 0;
     s.reject = s.iasti = s.nonsti = 0;
     double y[6], k1[6];
     FOR6 {
@@ -970,7 +974,7 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
     s.posneg = sign_d(1.0, s.xend - s.x);
     s.facold = 1.0e-4;
     s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
     s.reject = s.iasti = s.nonsti = 0;
@@ -1021,7 +1025,7 @@ __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ 
     s.posneg = sign_d(1.0, t1 - t0);
     s.facold = 1.0e-4;
     s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.nrejct = 0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
     s.reject = s.iasti = s.nonsti = 0;

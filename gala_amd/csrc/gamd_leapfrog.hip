@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp,
     double gx, gy, gz;
     pot_grad_l<SCF, LAY>(P, x, y, z, gx, gy, gz);
     double hx = vx - gx * dt / 2.0, hy = vy - gy * dt / 2.0, hz = vz - gz * dt / 2.0;
-    int64_t next_save = stride;
+    int64_t next_save = stride, o = i;
     for (int64_t j = 1; j <= ns; j++) {
         x = x + hx * dt;
         y = y + hy * dt;
@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp,
         hz = hz - gz * dt;
         if (SAVE && j == next_save) {
             next_save += stride;
-            const int64_t s = (int64_t)nsave * N, o = (j / stride) * N + i;
+            o += N;  // saved row j / stride, kept as a running offset (no 64-bit divide)
+            const int64_t s = (int64_t)nsave * N;
             wout[o] = x;
             wout[s + o] = y;
             wout[2 * s + o] = z;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_ruth4(const DevPot *__restrict__ Pp, co
         wout[4 * s + i] = vy;
         wout[5 * s + i] = vz;
     }
-    int64_t next_save = stride;
+    int64_t next_save = stride, o = i;
     for (int64_t j = 1; j <= n_steps; j++) {
         // substep 0: d_0 = 0, drift only
         x = x + cf.c[0] * vx * dt;
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(256) void k_ruth4(const DevPot *__restrict__ Pp, co
         }
         if (SAVE && j == next_save) {
             next_save += stride;
-            const int64_t s = (int64_t)nsave * N, o = (j / stride) * N + i;
+            o += N;  // saved row j / stride, kept as a running offset (no 64-bit divide)
+            const int64_t s = (int64_t)nsave * N;
             wout[o] = x;
             wout[s + o] = y;
             wout[2 * s + o] = z;

@@ -198,10 +198,15 @@ __global__ __launch_bounds__(64) void k_nbody_leapfrog(const DevPot *__restrict_
     if (SAVE) store(wout, nsave, 0);
     bodies_init<SCF, LAY, KMAX>(E, bt, K, dt, B);
     if (has_p) particle_init<SCF, LAY, KMAX>(E, bt, B, dt, p);
+    // the saved row s / stride as a running count (no 64-bit divide per step)
+    int64_t next_save = stride, row = 0;
     for (int64_t s = 1; s <= n_steps; s++) {
         bodies_step<SCF, LAY, KMAX>(E, bt, K, dt, B);
         if (has_p) particle_step<SCF, LAY, KMAX>(E, bt, B, dt, p);
-        if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
+        if (SAVE && s == next_save) {
+            next_save += stride;
+            store(wout, nsave, ++row);
+        }
     }
     if (wfinal) store(wfinal, 1, 0);
 }
@@ -292,12 +297,16 @@ __global__ __launch_bounds__(64) void k_nbody_ruth4(const DevPot *__restrict__ E
         }
     };
     if (SAVE) store(wout, nsave, 0);
+    int64_t next_save = stride, row = 0;  // saved row s / stride without a divide
     for (int64_t s = 1; s <= n_steps; s++) {
 #pragma unroll
         for (int i = 0; i < KMAX; i++)
             if (i < K) ruth4_substeps<SCF, LAY, KMAX>(E, bt, i, bx, dt, cf, bx[i], bv[i]);
         if (has_p) ruth4_substeps<SCF, LAY, KMAX>(E, bt, -1, bx, dt, cf, px, pv);
-        if (SAVE && s % stride == 0) store(wout, nsave, s / stride);
+        if (SAVE && s == next_save) {
+            next_save += stride;
+            store(wout, nsave, ++row);
+        }
     }
     if (wfinal) store(wfinal, 1, 0);
 }
