@@ -102,7 +102,7 @@ __device__ __forceinline__ void out_scales(const gamd_mlp_norm &nm, double r, do
 // activations of the four hidden layers go to global (BF16, [4][M][256]).
 template <bool TRAIN>
 __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, const __bf16 *__restrict__ wsw,
-                             const float (*feat)[4], __bf16 *bufA, __bf16 *bufB, float (*out)[4],
+                             const float (*feat)[4], __bf16 *buf, float (*out)[4],
                              int64_t row0, int64_t M, __bf16 *__restrict__ Zs,
                              __bf16 *__restrict__ Hs) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -116,7 +116,7 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
         for (int p = 0; p < TILE; p++) {
             const float z = bj + feat[p][0] * w0 + feat[p][1] * w1 + feat[p][2] * w2 + feat[p][3] * w3;
             const float h = silu(z);
-            bufA[p * LDSR + j] = tobf(h);
+            buf[p * LDSR + j] = tobf(h);
             if (TRAIN && row0 + p < M) {
                 Zs[(0 * M + row0 + p) * H + j] = tobf(z);
                 Hs[(0 * M + row0 + p) * H + j] = tobf(h);
@@ -125,46 +125,50 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
     }
     __syncthreads();
     // layers 2..4 on the matrix cores
-    __bf16 *in = bufA, *outb = bufB;
+    // one activation buffer: every wave holds its 64 units of the whole
+    // layer in accumulators, so the layer reads the buffer, waits for every
+    // wave, then overwrites it (68 KB of LDS: two workgroups per CU, whose
+    // matrix-core and SiLU phases overlap)
 #pragma unroll 1
     for (int l = 0; l < 3; l++) {
         const float *bias = theta + (l == 0 ? OFF_B2 : l == 1 ? OFF_B3 : OFF_B4);
         const __bf16 *W = wsw + (size_t)l * H * H;
-        // the wave's 64 columns in two passes of 32: 64 FP32 accumulators
-        // per lane instead of 128, the A fragments re-read from LDS
-#pragma unroll 1
-        for (int half = 0; half < 2; half++) {
-            f32x4 acc[8][2];
+        // the wave's 64 units in one pass (128 FP32 accumulators per lane):
+        // each activation fragment is read from LDS once per layer
+        {
+            constexpr int NT = 4;  // 16-unit tiles per wave
+            f32x4 acc[8][NT];
 #pragma unroll
             for (int mi = 0; mi < 8; mi++)
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int ni = 0; ni < NT; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
             for (int ks = 0; ks < 8; ks++) {
-                bf16x8 b[2];
+                bf16x8 b[NT];
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++)
+                for (int ni = 0; ni < NT; ni++)
                     b[ni] = *reinterpret_cast<const bf16x8 *>(
-                        W + (((4 * w + 2 * half + ni) * 8 + ks) * 64 + lane) * 8);
+                        W + (((4 * w + ni) * 8 + ks) * 64 + lane) * 8);
 #pragma unroll
                 for (int mi = 0; mi < 8; mi++) {
                     const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
-                        in + (16 * mi + (lane & 15)) * LDSR + 32 * ks + 8 * (lane >> 4));
+                        buf + (16 * mi + (lane & 15)) * LDSR + 32 * ks + 8 * (lane >> 4));
                     // the transposed product, Z^T = W^T H^T: the weight fragment
                     // is the A operand and the activations the B operand, so a
                     // lane's four accumulators are four consecutive units of
                     // one point (one 8-byte store below instead of four 2-byte)
 #pragma unroll
-                    for (int ni = 0; ni < 2; ni++)
+                    for (int ni = 0; ni < NT; ni++)
                         acc[mi][ni] =
                             __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[ni], a, acc[mi][ni], 0, 0, 0);
                 }
             }
+            __syncthreads();  // every wave has read the layer's input
 #pragma unroll
             for (int mi = 0; mi < 8; mi++)
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++) {
-                    const int unit = 64 * w + 16 * (2 * half + ni) + 4 * (lane >> 4);
+                for (int ni = 0; ni < NT; ni++) {
+                    const int unit = 64 * w + 16 * ni + 4 * (lane >> 4);
                     const int row = 16 * mi + (lane & 15);
                     const f32x4 bc = *reinterpret_cast<const f32x4 *>(bias + unit);
                     __bf16 hv[4], zv[4];
@@ -175,7 +179,7 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
                         zv[e] = tobf(z);
                     }
                     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                    *reinterpret_cast<bf16x4 *>(outb + row * LDSR + unit) =
+                    *reinterpret_cast<bf16x4 *>(buf + row * LDSR + unit) =
                         bf16x4{hv[0], hv[1], hv[2], hv[3]};
                     if (TRAIN && row0 + row < M) {
                         *reinterpret_cast<bf16x4 *>(Zs + ((size_t)(l + 1) * M + row0 + row) * H + unit) =
@@ -186,16 +190,13 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
                 }
         }
         __syncthreads();
-        __bf16 *tmp = in;
-        in = outb;
-        outb = tmp;
     }
     // layer 5: 256 -> 4 in FP32 from the BF16 activations; thread t: point
     // t/2, outputs 2(t%2), 2(t%2)+1
     {
         const int p = t >> 1, o0 = 2 * (t & 1);
         float s0 = theta[OFF_B5 + o0], s1 = theta[OFF_B5 + o0 + 1];
-        const __bf16 *hrow = in + p * LDSR;
+        const __bf16 *hrow = buf + p * LDSR;
 #pragma unroll 8
         for (int j = 0; j < H; j++) {
             const float hj = frombf(hrow[j]);
@@ -208,19 +209,18 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
     __syncthreads();
 }
 
-constexpr int LDS_BYTES = 2 * TILE * LDSR * 2;
+constexpr int LDS_BYTES = TILE * LDSR * 2;
 
 // ---------------------------------------------------------------------------
 // inference: q (3,M) FP64 -> Phi (M), grad (3,M)
-__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ theta,
+__global__ __launch_bounds__(256, 2) void k_mlp_forward(const float *__restrict__ theta,
                                                      const __bf16 *__restrict__ wsw,
                                                      gamd_mlp_norm nm,
                                                      const double *__restrict__ q, int64_t M,
                                                      double *__restrict__ phi,
                                                      double *__restrict__ grad) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
-    __bf16 *bufB = bufA + TILE * LDSR;
+    __bf16 *buf = reinterpret_cast<__bf16 *>(smem);
     __shared__ float feat[TILE][4];
     __shared__ float out[TILE][4];
     const int t = threadIdx.x;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ t
         feat[t][3] = f[3];
     }
     __syncthreads();
-    tile_forward<false>(theta, wsw, feat, bufA, bufB, out, row0, M, nullptr, nullptr);
+    tile_forward<false>(theta, wsw, feat, buf, out, row0, M, nullptr, nullptr);
     if (t < TILE) {
         const int64_t i = row0 + t;
         if (i < M) {
@@ -258,15 +258,14 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ t
 }
 
 // training forward: feat (M,4) FP32 -> Zs, Hs (BF16 [4][M][256]), Y (M,4)
-__global__ __launch_bounds__(256) void k_mlp_forward_train(const float *__restrict__ theta,
+__global__ __launch_bounds__(256, 2) void k_mlp_forward_train(const float *__restrict__ theta,
                                                            const __bf16 *__restrict__ wsw,
                                                            const float *__restrict__ featg,
                                                            int64_t M, __bf16 *__restrict__ Zs,
                                                            __bf16 *__restrict__ Hs,
                                                            float *__restrict__ Y) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
-    __bf16 *bufB = bufA + TILE * LDSR;
+    __bf16 *buf = reinterpret_cast<__bf16 *>(smem);
     __shared__ float feat[TILE][4];
     __shared__ float out[TILE][4];
     const int t = threadIdx.x;
@@ -277,7 +276,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward_train(const float *__restri
         for (int c = 0; c < 4; c++) feat[t][c] = i < M ? featg[i * 4 + c] : 0.0f;
     }
     __syncthreads();
-    tile_forward<true>(theta, wsw, feat, bufA, bufB, out, row0, M, Zs, Hs);
+    tile_forward<true>(theta, wsw, feat, buf, out, row0, M, Zs, Hs);
     if (t < TILE && row0 + t < M)
 #pragma unroll
         for (int c = 0; c < 4; c++) Y[(row0 + t) * 4 + c] = out[t][c];
@@ -532,7 +531,8 @@ __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
 }
 
 // ---------------------------------------------------------------------------
-// surrogate leapfrog, one launch per step.  Orbit i: w (6,N) holds x and the
+// surrogate leapfrog, one step per call (this kernel for the orbits on the
+// network, then k_surrogate_fallback for those on the exact force).  Orbit i: w (6,N) holds x and the
 // half-step velocity between launches (the synchronised velocity after the
 // last step); E0[i] the initial energy; fb[i] the step at which the orbit fell
 // back to exact SCF (-1 while on the surrogate).  mode 0: initial half kick;
@@ -540,17 +540,14 @@ __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
 // Phi + |v|^2/2 against E0; once |E - E0| > tol |E0| the orbit switches to
 // the exact SCF force for the rest of the run (and for the remainder of this
 // step).
-__global__ __launch_bounds__(256) void k_surrogate_step(
+__global__ __launch_bounds__(256, 2) void k_surrogate_step(
     const float *__restrict__ theta, const __bf16 *__restrict__ wsw, gamd_mlp_norm nm,
-    const DevPot *__restrict__ Pp, double *__restrict__ wq, double *__restrict__ vsync,
-    double *__restrict__ E0, int32_t *__restrict__ fb, int64_t N, double dt, int mode,
-    int32_t step, double tol) {
+    double *__restrict__ wq, double *__restrict__ vsync, double *__restrict__ E0,
+    int32_t *__restrict__ fb, int64_t N, double dt, int mode, int32_t step, double tol) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *bufA = reinterpret_cast<__bf16 *>(smem);
-    __bf16 *bufB = bufA + TILE * LDSR;
+    __bf16 *buf = reinterpret_cast<__bf16 *>(smem);
     __shared__ float feat[TILE][4];
     __shared__ float out[TILE][4];
-    const DevPot &P = *Pp;
     const int t = threadIdx.x;
     const int64_t row0 = (int64_t)blockIdx.x * TILE;
     const int64_t i = row0 + t;
@@ -582,7 +579,7 @@ __global__ __launch_bounds__(256) void k_surrogate_step(
     __syncthreads();
     // the tile runs the network unless every orbit in it has fallen back
     const int any_net = __syncthreads_or(own && fallback < 0);
-    if (any_net) tile_forward<false>(theta, wsw, feat, bufA, bufB, out, row0, N, nullptr, nullptr);
+    if (any_net) tile_forward<false>(theta, wsw, feat, buf, out, row0, N, nullptr, nullptr);
     if (!own) return;
     double gx, gy, gz, phi;
     if (fallback < 0) {
@@ -595,7 +592,6 @@ __global__ __launch_bounds__(256) void k_surrogate_step(
     }
     const double vx0 = (mode == 0) ? hx : 0.0, vy0 = (mode == 0) ? hy : 0.0,
                  vz0 = (mode == 0) ? hz : 0.0;
-    bool switch_now = false;
     if (fallback < 0) {
         // energy monitor on the synchronised velocity this force gives
         const double vx = (mode == 0) ? vx0 : hx - gx * dt / 2.0;
@@ -605,18 +601,51 @@ __global__ __launch_bounds__(256) void k_surrogate_step(
         if (mode == 0) {
             E0[i] = E;
         } else if (fabs(E - E0[i]) > tol * fabs(E0[i])) {
-            switch_now = true;
             fallback = step;
             fb[i] = step;
         }
     }
-    if (fallback >= 0) {
-        double ph;
-        gx = gy = gz = 0.0;
-        const DevComp &c = P.c[0];
-        scf_eval(c, x, y, z, true, ph, gx, gy, gz);
-        (void)switch_now;
+    // orbits on the exact force (already, or from this step on) are stepped
+    // by k_surrogate_fallback, launched right after on the same stream
+    if (fallback >= 0) return;
+    if (mode == 0) {
+        wq[3 * N + i] = hx - gx * dt / 2.0;
+        wq[4 * N + i] = hy - gy * dt / 2.0;
+        wq[5 * N + i] = hz - gz * dt / 2.0;
+    } else {
+        wq[i] = x;
+        wq[N + i] = y;
+        wq[2 * N + i] = z;
+        vsync[i] = hx - gx * dt / 2.0;
+        vsync[N + i] = hy - gy * dt / 2.0;
+        vsync[2 * N + i] = hz - gz * dt / 2.0;
+        wq[3 * N + i] = hx - gx * dt;
+        wq[4 * N + i] = hy - gy * dt;
+        wq[5 * N + i] = hz - gz * dt;
     }
+}
+
+// The same leapfrog step for the orbits on the exact SCF force (fb >= 0,
+// including those k_surrogate_step has just switched): the drift is redone
+// from the stored state with the same arithmetic, then the force, kick and
+// writes as k_surrogate_step would make them.  A kernel of its own keeps the
+// SCF evaluator's registers out of the network kernel.
+__global__ __launch_bounds__(256) void k_surrogate_fallback(const DevPot *__restrict__ Pp,
+                                                            double *__restrict__ wq,
+                                                            double *__restrict__ vsync,
+                                                            const int32_t *__restrict__ fb,
+                                                            int64_t N, double dt, int mode) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= N || fb[i] < 0) return;
+    double x = wq[i], y = wq[N + i], z = wq[2 * N + i];
+    const double hx = wq[3 * N + i], hy = wq[4 * N + i], hz = wq[5 * N + i];
+    if (mode == 1) {
+        x = x + hx * dt;
+        y = y + hy * dt;
+        z = z + hz * dt;
+    }
+    double ph, gx = 0.0, gy = 0.0, gz = 0.0;
+    scf_eval(Pp->c[0], x, y, z, true, ph, gx, gy, gz);
     if (mode == 0) {
         wq[3 * N + i] = hx - gx * dt / 2.0;
         wq[4 * N + i] = hy - gy * dt / 2.0;
@@ -646,8 +675,8 @@ int check_params(const float *theta, const uint16_t *wbf) {
     return (theta && wbf) ? GAMD_OK : GAMD_ERR_INVALID;
 }
 
-// the tile kernels use 132 KB of dynamic LDS (two padded 128 x 256 BF16
-// activation buffers); raise their limit once per process
+// the tile kernels use 66 KB of dynamic LDS (one padded 128 x 256 BF16
+// activation buffer), above the default 64 KB; raise their limit once per process
 int ensure_lds() {
     static bool done = false;
     if (done) return GAMD_OK;
@@ -788,10 +817,14 @@ int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const 
     if (mode != 0 && mode != 1) return GAMD_ERR_INVALID;
     if (N == 0) return GAMD_OK;
     if (ensure_lds()) return GAMD_ERR_HIP;
+    hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_surrogate_step, dim3((unsigned)((N + TILE - 1) / TILE)), dim3(256),
-                       LDS_BYTES, as_stream(stream), theta, reinterpret_cast<const __bf16 *>(wbf),
-                       *norm, scf->ddev, w, vsync, E0, fallback_step, (int64_t)N, dt, mode, step,
-                       tol);
+                       LDS_BYTES, s, theta, reinterpret_cast<const __bf16 *>(wbf), *norm, w, vsync,
+                       E0, fallback_step, (int64_t)N, dt, mode, step, tol);
+    int rc = launch_status();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_surrogate_fallback, dim3(grid_for(N, 256)), dim3(256), 0, s, scf->ddev, w,
+                       vsync, fallback_step, (int64_t)N, dt, mode);
     return launch_status();
 }
 
