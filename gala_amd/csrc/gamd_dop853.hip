@@ -344,8 +344,8 @@ struct QuadBodies {
 template <bool SCF, int LAY>
 struct RowBodies {
     static constexpr int NC = 2;
-    int body, axis, nb;   // axis 0..2 (lane 3 of a quad runs axis 0 again)
-    const BodyTable &bt;  // bt.n <= nb <= 4
+    int body, axis, lane4, nb;  // axis 0..2 (lane4 = 3, the quad's last lane, runs axis 0 again)
+    const BodyTable &bt;        // bt.n <= nb <= 4
     template <int J>
     __device__ __forceinline__ void add_body(const double (&y)[2], double qx, double qy, double qz,
                                              double &gx, double &gy, double &gz) const {
@@ -396,6 +396,15 @@ struct RowBodies {
         sums_body<3>(a, b, sa, sb);
     }
     __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
+    // dense sample s.oidx of body `body` into w_out (6, n_out, nb)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[2]) const {
+        const int64_t so = a.n_out * a.N;
+        if (body < nb && lane4 < 3) {
+            a.w_out[axis * so + s.oidx * a.N + body] = v[0];
+            a.w_out[(axis + 3) * so + s.oidx * a.N + body] = v[1];
+        }
+    }
 };
 
 // v of lane src (wave-uniform) in every lane
@@ -459,7 +468,6 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                                            DopState &s, double (&y)[Sys::NC],
                                            double (&k1)[Sys::NC]) {
     constexpr int NC = Sys::NC;
-    static_assert(NC == 6 || !DENSE, "dense output runs with one lane per orbit");
     const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
     if (s.nstep > a.nmax) return -2;
     if (0.1 * fabs(s.h) <= fabs(s.x) * a.uround) return -3;
@@ -557,8 +565,8 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
         if (code == 0) {
             if constexpr (DENSE) {
                 // dense output coefficients (:492-582) and sampling (:584-612)
-                double r1[6], r2[6], r3[6], r4[6], r5[6], r6[6], r7[6], r8[6];
-                FOR6 {
+                double r1[NC], r2[NC], r3[NC], r4[NC], r5[NC], r6[NC], r7[NC], r8[NC];
+                FORN {
                     r1[c] = y[c];
                     const double ydiff = k5[c] - y[c];
                     r2[c] = ydiff;
@@ -578,23 +586,23 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                             T::D[3][7] * k8[c] + T::D[3][8] * k9[c] + T::D[3][9] * k10[c] +
                             T::D[3][10] * k2[c] + T::D[3][11] * k3[c];
                 }
-                FOR6 yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
+                FORN yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
                                          T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
                                          T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
                                          T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
                 sys.rhs(P, yy, k10);
-                FOR6 yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
+                FORN yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
                                          T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
                                          T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
                                          T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
                 sys.rhs(P, yy, k2);
-                FOR6 yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
+                FORN yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
                                          T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
                                          T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
                                          T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
                 sys.rhs(P, yy, k3);
                 s.nfcn += 3;
-                FOR6 {
+                FORN {
                     r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
                                  T::D[0][14] * k2[c] + T::D[0][15] * k3[c]);
                     r6[c] = h * (r6[c] + T::D[1][12] * k4[c] + T::D[1][13] * k10[c] +
@@ -609,8 +617,8 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                     const double to = a.t_out[s.oidx];
                     if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
                         const double sv = (to - s.x) / h, s1 = 1.0 - sv;
-                        double v[6];
-                        FOR6 v[c] = r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
+                        double v[NC];
+                        FORN v[c] = r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
                                     sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
                         sys.store_dense(a, s, v);
                         s.oidx++;
@@ -945,7 +953,7 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
     constexpr int NC = Sys::NC;
     auto make_sys = [&]() {
         if constexpr (G == 16)
-            return Sys{body, axis, nb, bt};
+            return Sys{body, axis, lane4, nb, bt};
         else
             return Sys{body, nb, bt};
     };
@@ -1005,17 +1013,36 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
 }
 
 // direct_nbody_dop853 (nbody.pyx:30-115): the a.N bodies of w0 (6, N) as one
-// system on one wave, dense output at t_out (dop853_helper, dop853.pyx:90-193).
-template <bool SCF, bool DENSE, int LAY>
+// system, dense output at t_out (dop853_helper, dop853.pyx:90-193).  G = 64:
+// one wave, body b on lane b (WaveBodies); G = 16: at most 4 bodies on one
+// 16-lane row (RowBodies, the other rows of the wave exit at once).
+template <bool SCF, bool DENSE, int LAY, int G>
 __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ Pp, BodyTable bt,
                                                      double t0, double t1, double h0, DopArgs a) {
     const DevPot &P = *Pp;
-    const int body = threadIdx.x;
+    const int t = threadIdx.x;
+    if (t >= G) return;
+    const int body = G == 16 ? t / 4 : t;
+    const int lane4 = t % 4;
+    const int axis = lane4 < 3 ? lane4 : 0;
     const int nb = (int)a.N;
-    const WaveBodies<SCF, LAY> sys{body, nb, bt};
+    using Sys = typename std::conditional<G == 16, RowBodies<SCF, LAY>, WaveBodies<SCF, LAY>>::type;
+    constexpr int NC = Sys::NC;
+    auto make_sys = [&]() {
+        if constexpr (G == 16)
+            return Sys{body, axis, lane4, nb, bt};
+        else
+            return Sys{body, nb, bt};
+    };
+    const Sys sys = make_sys();
     const int src = body < nb ? body : 0;
-    double y[6], k1[6];
-    FOR6 y[c] = a.w0[c * a.N + src];
+    double y[NC], k1[NC];
+    if constexpr (NC == 6) {
+        FOR6 y[c] = a.w0[c * a.N + src];
+    } else {
+        y[0] = a.w0[axis * a.N + src];
+        y[1] = a.w0[(axis + 3) * a.N + src];
+    }
     DopState s;
     s.orb = 0;
     s.x = t0;
@@ -1031,10 +1058,30 @@ __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ 
     s.reject = s.iasti = s.nonsti = 0;
     sys.rhs(P, y, k1);
     int code = 0;
-    // the wave ends: each attempt advances nstep, and nmax bounds it
+    // the group ends: each attempt advances nstep, and nmax bounds it
     while (code == 0) code = dop_attempt<SCF, DENSE, LAY>(P, a, sys, s, y, k1);
-    if (body < nb) FOR6 a.w_final[c * a.N + body] = y[c];
-    if (body == 0) finish_stats(a, s, code);
+    if (body < nb) {
+        if constexpr (NC == 6) {
+            FOR6 a.w_final[c * a.N + body] = y[c];
+        } else if (lane4 < 3) {
+            a.w_final[axis * a.N + body] = y[0];
+            a.w_final[(axis + 3) * a.N + body] = y[1];
+        }
+    }
+    if (t == 0) finish_stats(a, s, code);
+}
+
+// Systems of at most 4 bodies on one 16-lane row (two components per lane)
+// unless GAMD_DOP853_BODY_ROWS=0; =1 forces rows wherever they fit.  Same
+// results either way.
+bool body_rows(int64_t systems, int nb) {
+    if (nb > 4) return false;
+    const char *env = getenv("GAMD_DOP853_BODY_ROWS");
+    if (env && env[0] == '0') return false;
+    if (env && env[0] == '1') return true;
+    // rows while they fit about two waves per SIMD (the launch is then bound
+    // by its longest systems, whose attempts rows make shorter)
+    return 16 * systems <= 2 * 64 * 4 * (int64_t)cu_count();
 }
 
 template <bool SCF, bool DENSE, int LAY>
@@ -1165,15 +1212,9 @@ extern "C" int gamd_mockstream_dop853(const gamd_potential *ext,
     const int block = 256;
     hipStream_t s = as_stream(stream);
     if (quads) {
-        // Systems of at most 4 bodies: one per 16-lane row while the rows fit
-        // about two waves per SIMD (the launch is then bound by its longest
-        // systems, the earliest releases, whose attempts rows make shorter),
-        // else one per quad (a quarter of the lanes per system).
-        // GAMD_DOP853_STREAM_ROWS=0 / =1 forces quads / rows (same results).
-        const char *rows_env = getenv("GAMD_DOP853_STREAM_ROWS");
-        bool rows = 16 * (int64_t)nrel <= 2 * 64 * 4 * (int64_t)cu_count();
-        if (rows_env && rows_env[0] == '0') rows = false;
-        if (rows_env && rows_env[0] == '1') rows = true;
+        // Systems of at most 4 bodies: one per 16-lane row (body_rows) or one
+        // per quad (a quarter of the lanes per system).
+        const bool rows = body_rows((int64_t)nrel, 4);
         const int G = rows ? 16 : 4;
         const unsigned grid = (unsigned)((G * (int64_t)nrel + block - 1) / block);
         auto kern = rows ? (scf  ? k_dop853_stream<true, LAYOUT_GENERIC, 16>
@@ -1234,12 +1275,19 @@ extern "C" int gamd_nbody_dop853(const gamd_potential *ext, const gamd_potential
     const bool scf = ext->dev.has_scf || bt.scf;
     const bool mw = ext->dev.layout == LAYOUT_MW && !bt.scf;
     hipStream_t s = as_stream(stream);
-    auto kern = w_out ? (scf  ? k_dop853_nbody<true, true, LAYOUT_GENERIC>
-                         : mw ? k_dop853_nbody<false, true, LAYOUT_MW>
-                              : k_dop853_nbody<false, true, LAYOUT_GENERIC>)
-                      : (scf  ? k_dop853_nbody<true, false, LAYOUT_GENERIC>
-                         : mw ? k_dop853_nbody<false, false, LAYOUT_MW>
-                              : k_dop853_nbody<false, false, LAYOUT_GENERIC>);
+    auto kern = body_rows(1, (int)n)
+                    ? (w_out ? (scf  ? k_dop853_nbody<true, true, LAYOUT_GENERIC, 16>
+                                : mw ? k_dop853_nbody<false, true, LAYOUT_MW, 16>
+                                     : k_dop853_nbody<false, true, LAYOUT_GENERIC, 16>)
+                             : (scf  ? k_dop853_nbody<true, false, LAYOUT_GENERIC, 16>
+                                : mw ? k_dop853_nbody<false, false, LAYOUT_MW, 16>
+                                     : k_dop853_nbody<false, false, LAYOUT_GENERIC, 16>))
+                    : (w_out ? (scf  ? k_dop853_nbody<true, true, LAYOUT_GENERIC, 64>
+                                : mw ? k_dop853_nbody<false, true, LAYOUT_MW, 64>
+                                     : k_dop853_nbody<false, true, LAYOUT_GENERIC, 64>)
+                             : (scf  ? k_dop853_nbody<true, false, LAYOUT_GENERIC, 64>
+                                : mw ? k_dop853_nbody<false, false, LAYOUT_MW, 64>
+                                     : k_dop853_nbody<false, false, LAYOUT_GENERIC, 64>));
     hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, s, ext->ddev, bt, t0, t1, h0, a);
     return launch_status();
 }

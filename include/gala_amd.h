@@ -224,9 +224,10 @@ int gamd_mockstream_leapfrog(const gamd_potential *ext, const gamd_potential *co
  * leaves its rows of body_final and stats untouched); stats (nrel, 4)
  * nullable.  Systems of at most 4 bodies run one per 16-lane row (up to
  * 2 x 64 x 4 x CUs / 16 systems) or one per 4-lane quad (more systems);
- * environment GAMD_DOP853_STREAM_ROWS=0 / =1 forces quads / rows, with
- * identical results.  GAMD_DOP853_WIDE=0 / =1 likewise switches off
- * gamd_dop853's 16-lane / 4-lane tail stages (identical results). */
+ * gamd_nbody_dop853 likewise runs at most 4 bodies on one row, more on one
+ * wave.  Environment GAMD_DOP853_BODY_ROWS=0 / =1 forces one body per lane /
+ * rows, with identical results.  GAMD_DOP853_WIDE=0 / =1 likewise switches
+ * off gamd_dop853's 16-lane / 4-lane tail stages (identical results). */
 int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *const *bodies,
                            int nbody, const double *body_w, size_t nrel, const double *rel_t,
                            const int32_t *nstream, const int64_t *offset, const double *stream_w0,

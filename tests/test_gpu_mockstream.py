@@ -239,7 +239,7 @@ def test_dop853_release_rows_match_quads(with_perturber, monkeypatch):
     """Release systems of at most 4 bodies run one per 16-lane row (a body
     per quad, an axis per lane) or one per 4-lane quad (a body per lane); the
     two layouts must give bit-identical states, statuses and step counts.
-    GAMD_DOP853_STREAM_ROWS=0 forces quads, =1 rows."""
+    GAMD_DOP853_BODY_ROWS=0 forces quads, =1 rows."""
     from gala_amd import _device as dev
     from gala_amd import _lib
     from gala_amd.potential import HernquistPotential, MilkyWayLikePotential, PlummerPotential
@@ -262,7 +262,7 @@ def test_dop853_release_rows_match_quads(with_perturber, monkeypatch):
     lib = _lib.load()
     res = {}
     for mode in ("0", "1"):
-        monkeypatch.setenv("GAMD_DOP853_STREAM_ROWS", mode)
+        monkeypatch.setenv("GAMD_DOP853_BODY_ROWS", mode)
         out = t.empty((ntot, 6), dtype=t.float64, device="cuda")
         bf = t.empty((nrel, K, 6), dtype=t.float64, device="cuda")
         status = t.empty(nrel, dtype=t.int32, device="cuda")
