@@ -676,18 +676,20 @@ int check_params(const float *theta, const uint16_t *wbf) {
 }
 
 // the tile kernels use 66 KB of dynamic LDS (one padded 128 x 256 BF16
-// activation buffer), above the default 64 KB; raise their limit once per process
+// activation buffer), above the default 64 KB; raise their limit once per
+// process (a function-local static: initialised once even when several host
+// threads make their first call together)
 int ensure_lds() {
-    static bool done = false;
-    if (done) return GAMD_OK;
-    const void *ks[3] = {(const void *)k_mlp_forward, (const void *)k_mlp_forward_train,
-                         (const void *)k_surrogate_step};
-    for (const void *k : ks)
-        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES) !=
-            hipSuccess)
-            return GAMD_ERR_HIP;
-    done = true;
-    return GAMD_OK;
+    static const int rc = [] {
+        const void *ks[3] = {(const void *)k_mlp_forward, (const void *)k_mlp_forward_train,
+                             (const void *)k_surrogate_step};
+        for (const void *k : ks)
+            if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES) !=
+                hipSuccess)
+                return (int)GAMD_ERR_HIP;
+        return (int)GAMD_OK;
+    }();
+    return rc;
 }
 
 }  // namespace
