@@ -353,6 +353,81 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const __bf16 
             }
 }
 
+// Weight gradient of a hidden layer, C[m][n] += sum_p A[p][m] B[p][n] over
+// this workgroup's slice of the P points (split over gridDim.z), with A and B
+// row-major [P][256] BF16 (a layer's input activations and its pre-activation
+// gradients) and C [256][256] FP32 (atomic adds across slices; the caller
+// zeroes it).  The reduction runs along the points, which both operands store
+// 512 bytes apart, so tiles of 64 points x 64 columns are read with coalesced
+// 16-byte loads and written transposed into LDS: each matrix-core fragment (8
+// consecutive points of one column) is then one 16-byte LDS read.  64 x 64
+// outputs per workgroup, a 32 x 32 quarter per wave.
+constexpr int WG_KC = 64;          // points per LDS stage
+constexpr int WG_LDK = WG_KC + 8;  // padded LDS row in BF16 (144 bytes, 16-byte aligned)
+
+__global__ __launch_bounds__(256) void k_wgrad(int P, const __bf16 *__restrict__ A,
+                                               const __bf16 *__restrict__ B,
+                                               float *__restrict__ C) {
+    __shared__ __align__(16) __bf16 As[64][WG_LDK];
+    __shared__ __align__(16) __bf16 Bs[64][WG_LDK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int mt = blockIdx.x * 64, nt = blockIdx.y * 64;  // the workgroup's output tile
+    const int mw = 32 * (w >> 1), nw = 32 * (w & 1);       // the wave's quarter of it
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int li = lane & 15, lk = 8 * (lane >> 4);
+    const int pslice = P / (int)gridDim.z;
+    const int p_lo = (int)blockIdx.z * pslice, p_hi = p_lo + pslice;
+    // staging: thread t loads points r and r + 32 of the stage, columns c8 .. c8 + 7
+    const int r = t >> 3, c8 = 8 * (t & 7);
+    for (int p0 = p_lo; p0 < p_hi; p0 += WG_KC) {
+        bf16x8 va[2], vb[2];
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int64_t p = (int64_t)p0 + r + 32 * hh;
+            va[hh] = *reinterpret_cast<const bf16x8 *>(A + p * H + mt + c8);
+            vb[hh] = *reinterpret_cast<const bf16x8 *>(B + p * H + nt + c8);
+        }
+        __syncthreads();  // every wave has read the previous stage
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                As[c8 + e][r + 32 * hh] = va[hh][e];
+                Bs[c8 + e][r + 32 * hh] = vb[hh][e];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < WG_KC; ks += 32) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[mw + 16 * i + li][ks + lk]);
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[nw + 16 * j + li][ks + lk]);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = mt + mw + 16 * i + 4 * (lane >> 4) + e;
+                const int col = nt + nw + 16 * j + li;
+                atomicAdd(&C[row * H + col], acc[i][j][e]);
+            }
+}
+
 // loss and dY = 2 (Y - T) / M; per-block partial loss sums
 __global__ void k_loss(const float *__restrict__ Y, const float *__restrict__ T, int64_t M,
                        float *__restrict__ dY, float *__restrict__ partial) {
@@ -771,10 +846,11 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
     const int offW[3] = {OFF_W2, OFF_W3, OFF_W4}, offB[3] = {OFF_B2, OFF_B3, OFF_B4};
     for (int l = 2; l >= 0; l--) {  // l indexes W_{l+2}
         const __bf16 *Hin = Hs + (size_t)l * Mi * H;   // H_{l+1}
-        const unsigned ksplit = (Mi % (32 * 16) == 0) ? 16 : 1;  // grad is zeroed above
-        hipLaunchKernelGGL(k_gemm<0>, dim3(H / 64, H / 64, ksplit), dim3(256), 0, s, H, H, (int)Mi, Hin,
-                           (int64_t)1, (int64_t)H, dZ, (int64_t)H, (int64_t)1, grad + offW[l],
-                           (__bf16 *)nullptr, (int64_t)H, (const __bf16 *)nullptr);
+        // split the points over up to 16 slices of whole 64-point stages (grad is zeroed above)
+        unsigned ksplit = 16;
+        while (ksplit > 1 && Mi % (WG_KC * (int64_t)ksplit) != 0) ksplit /= 2;
+        hipLaunchKernelGGL(k_wgrad, dim3(H / 64, H / 64, ksplit), dim3(256), 0, s, (int)Mi, Hin, dZ,
+                           grad + offW[l]);
         hipLaunchKernelGGL(k_colsum, dim3(cs_blocks), dim3(256), 0, s, dZ, Mi, grad + offB[l]);
         hipLaunchKernelGGL((k_gemm<1, true>), dim3((unsigned)(Mi / 64), H / 64), dim3(256), 0, s, (int)Mi, H,
                            H, dZ, (int64_t)H, (int64_t)1, wrow + (size_t)l * H * H, (int64_t)1,
