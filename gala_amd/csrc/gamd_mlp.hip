@@ -361,15 +361,20 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const __bf16 
 // 512 bytes apart, so tiles of 64 points x 64 columns are read with coalesced
 // 16-byte loads and written transposed into LDS: each matrix-core fragment (8
 // consecutive points of one column) is then one 16-byte LDS read.  64 x 64
-// outputs per workgroup, a 32 x 32 quarter per wave.
+// outputs per workgroup, a 32 x 32 quarter per wave.  With gb, the
+// workgroups of the first row tile also add the column sums of B (the bias
+// gradient) into gb from the tiles they stage.
 constexpr int WG_KC = 64;          // points per LDS stage
 constexpr int WG_LDK = WG_KC + 8;  // padded LDS row in BF16 (144 bytes, 16-byte aligned)
 
 __global__ __launch_bounds__(256) void k_wgrad(int P, const __bf16 *__restrict__ A,
                                                const __bf16 *__restrict__ B,
-                                               float *__restrict__ C) {
+                                               float *__restrict__ C, float *__restrict__ gb) {
     __shared__ __align__(16) __bf16 As[64][WG_LDK];
     __shared__ __align__(16) __bf16 Bs[64][WG_LDK];
+    __shared__ float red[4][64];
+    const bool colsum = gb != nullptr && blockIdx.x == 0;  // uniform over the workgroup
+    float cs = 0.f;  // column t % 64, points 16 (t / 64) .. + 15 of each stage
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int mt = blockIdx.x * 64, nt = blockIdx.y * 64;  // the workgroup's output tile
     const int mw = 32 * (w >> 1), nw = 32 * (w & 1);       // the wave's quarter of it
@@ -400,6 +405,15 @@ __global__ __launch_bounds__(256) void k_wgrad(int P, const __bf16 *__restrict__
                 Bs[c8 + e][r + 32 * hh] = vb[hh][e];
             }
         __syncthreads();
+        if (colsum) {
+            const bf16x8 *src = reinterpret_cast<const bf16x8 *>(&Bs[t & 63][16 * (t >> 6)]);
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const bf16x8 v = src[q];
+#pragma unroll
+                for (int e = 0; e < 8; e++) cs += frombf(v[e]);
+            }
+        }
 #pragma unroll
         for (int ks = 0; ks < WG_KC; ks += 32) {
             bf16x8 a[2], b[2];
@@ -426,6 +440,11 @@ __global__ __launch_bounds__(256) void k_wgrad(int P, const __bf16 *__restrict__
                 const int col = nt + nw + 16 * j + li;
                 atomicAdd(&C[row * H + col], acc[i][j][e]);
             }
+    if (colsum) {
+        red[t >> 6][t & 63] = cs;
+        __syncthreads();
+        if (t < 64) atomicAdd(&gb[nt + t], red[0][t] + red[1][t] + red[2][t] + red[3][t]);
+    }
 }
 
 // loss and dY = 2 (Y - T) / M; per-block partial loss sums
@@ -850,8 +869,7 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
         unsigned ksplit = 16;
         while (ksplit > 1 && Mi % (WG_KC * (int64_t)ksplit) != 0) ksplit /= 2;
         hipLaunchKernelGGL(k_wgrad, dim3(H / 64, H / 64, ksplit), dim3(256), 0, s, (int)Mi, Hin, dZ,
-                           grad + offW[l]);
-        hipLaunchKernelGGL(k_colsum, dim3(cs_blocks), dim3(256), 0, s, dZ, Mi, grad + offB[l]);
+                           grad + offW[l], grad + offB[l]);
         hipLaunchKernelGGL((k_gemm<1, true>), dim3((unsigned)(Mi / 64), H / 64), dim3(256), 0, s, (int)Mi, H,
                            H, dZ, (int64_t)H, (int64_t)1, wrow + (size_t)l * H * H, (int64_t)1,
                            (int64_t)H, (float *)nullptr, dZn, (int64_t)H, Zs + (size_t)l * Mi * H);
