@@ -94,7 +94,7 @@ def bench_dop853(args):
         log(f"autotuned: block {block}, n_tail {n_tail}")
 
     def one(mid=None):
-        # pilot pass (2 % of every orbit's time span) -> longest-first order ->
+        # pilot pass (1 % of every orbit's time span) -> longest-first order ->
         # the full pass, all on the device; `mid` marks the full launch's start
         o = pilot_order(pot, w0d, t0d, t1d, h0d, rtol=1e-8, atol=1e-10, block=block)
         if mid is not None:
@@ -146,7 +146,7 @@ def bench_dop853(args):
                    "rtol": 1e-8, "atol": 1e-10, "T": "logU(0.1, 10) Gyr per orbit",
                    "r0": "logU(1, 200) kpc", "step": "one integration of every orbit",
                    "orbit_step": "accepted DOP853 step", "parallelism": "single GPU",
-                   "queue": f"longest first by a pilot pass over 2 % of each orbit's time "
+                   "queue": f"longest first by a pilot pass over 1 % of each orbit's time "
                             f"(inside the timed pass), {n_tail} on dedicated 16-lane rows",
                    "block": block,
                    "autotune_ms": None if tuned is None else tuned["trials_ms"]},
