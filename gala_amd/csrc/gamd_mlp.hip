@@ -538,10 +538,16 @@ __global__ __launch_bounds__(256) void k_colsum(const __bf16 *__restrict__ dZ, i
     atomicAdd(&gb[t], tot);
 }
 
-// first-layer weight gradient: gW1[c][j] += sum_m feat[m][c] dZ1[m][j]
-__global__ __launch_bounds__(256) void k_w1_grad(const float *__restrict__ feat,
-                                                 const __bf16 *__restrict__ dZ1, int64_t M,
-                                                 float *__restrict__ gW1) {
+// Outer-product reduction of an (M x 4) FP32 and an (M x 256) BF16 matrix,
+// g[c][j] += sum_m feat[m][c] dZ1[m][j]: the first layer's weight gradient
+// (UNIT_MAJOR false, stored [4][256]) and the output layer's (feat = dY,
+// dZ1 = the last hidden activations, UNIT_MAJOR true, stored [256][4]).
+// Row-blocked like k_colsum: thread t reads 8 adjacent columns of every 8th
+// row with one 16-byte load.
+template <bool UNIT_MAJOR>
+__global__ __launch_bounds__(256) void k_outer4(const float *__restrict__ feat,
+                                                const __bf16 *__restrict__ dZ1, int64_t M,
+                                                float *__restrict__ gW1) {
     __shared__ float red[8][4][H];
     const int t = threadIdx.x, c8 = 8 * (t & 31), rg = t >> 5;
     const int64_t r0 = (int64_t)blockIdx.x * CS_ROWS;
@@ -570,7 +576,7 @@ __global__ __launch_bounds__(256) void k_w1_grad(const float *__restrict__ feat,
         float tot = 0.f;
 #pragma unroll
         for (int g = 0; g < 8; g++) tot += red[g][c][t];
-        atomicAdd(&gW1[c * H + t], tot);
+        atomicAdd(&gW1[UNIT_MAJOR ? t * 4 + c : c * H + t], tot);
     }
 }
 
@@ -853,14 +859,14 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
     const unsigned lblocks = 512;
     hipLaunchKernelGGL(k_loss, dim3(lblocks), dim3(256), 0, s, Y, target, Mi, dY, partial);
     // output layer
-    hipLaunchKernelGGL(k_outer_reduce, dim3(H, 16), dim3(256), 0, s, Hs + 3 * Mi * H, H, dY, 4, Mi,
+    const unsigned cs_blocks = (unsigned)((Mi + CS_ROWS - 1) / CS_ROWS);
+    hipLaunchKernelGGL(k_outer4<true>, dim3(cs_blocks), dim3(256), 0, s, dY, Hs + 3 * Mi * H, Mi,
                        grad + OFF_W5);
     hipLaunchKernelGGL(k_outer_reduce, dim3(1, 16), dim3(256), 0, s, (const __bf16 *)nullptr, 1, dY,
                        4, Mi, grad + OFF_B5);
     hipLaunchKernelGGL(k_out_backward_dz, dim3(grid_for(Mi * H, 256)), dim3(256), 0, s, theta, dY,
                        Zs + 3 * Mi * H, Mi, dZa);
     // hidden layers 4, 3, 2: dW_l = H_{l-1}^T dZ_l, db_l, dZ_{l-1} = (dZ_l W_l^T) silu'(Z_{l-1})
-    const unsigned cs_blocks = (unsigned)((Mi + CS_ROWS - 1) / CS_ROWS);
     __bf16 *dZ = dZa, *dZn = dZb;
     const int offW[3] = {OFF_W2, OFF_W3, OFF_W4}, offB[3] = {OFF_B2, OFF_B3, OFF_B4};
     for (int l = 2; l >= 0; l--) {  // l indexes W_{l+2}
@@ -878,7 +884,8 @@ int gamd_mlp_gradients(const float *theta, const uint16_t *wbf, const float *fea
         dZn = tmp;
     }
     // first layer
-    hipLaunchKernelGGL(k_w1_grad, dim3(cs_blocks), dim3(256), 0, s, feat, dZ, Mi, grad + OFF_W1);
+    hipLaunchKernelGGL(k_outer4<false>, dim3(cs_blocks), dim3(256), 0, s, feat, dZ, Mi,
+                       grad + OFF_W1);
     hipLaunchKernelGGL(k_colsum, dim3(cs_blocks), dim3(256), 0, s, dZ, Mi, grad + OFF_B1);
     if (loss) {
         // sum of the per-block partials into loss[0]
