@@ -459,6 +459,104 @@ struct WaveBodies {
     }
 };
 
+// gala's default batching (dop853_integrate_hamiltonian, dop853.pyx:228-245):
+// the nb orbits of a batch form one (6, nb) system with one shared adaptive
+// step.  Orbit j of the batch lives on thread j of one workgroup (its six
+// components in VGPRs, as OneLane).  The error-norm and stiffness sums run
+// over the system's 6 nb components in the order of the (6, nb) state
+// vector, component-major, as dop853.cpp:416-444 and :463-486 loop over it:
+// every lane stages its six squared terms in LDS, rounded as the reference
+// rounds sqr*sqr, and every lane then adds all 6 nb terms in that order, so
+// the workgroup's lanes reach the same sums and take the same step decision.
+// Lanes past nb mirror orbit 0 of the batch and stay out of the sums.
+template <bool SCF, int LAY>
+struct BatchLanes {
+    static constexpr int NC = 6;
+    int j, nb;
+    int64_t base;  // the batch's first orbit
+    double *lds;   // 12 nb doubles: the a-terms, then the b-terms
+    __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
+        orbit_rhs<SCF, LAY>(P, y, f);
+    }
+    __device__ __forceinline__ void sums(const double (&a)[6], const double (&b)[6], double &sa,
+                                         double &sb) const {
+        const int n = 6 * nb;
+        __syncthreads();  // every lane has read the previous sums' terms
+        if (j < nb) {
+            FOR6 {
+                const double pa = a[c] * a[c], pb = b[c] * b[c];
+                lds[c * nb + j] = pa;
+                lds[n + c * nb + j] = pb;
+            }
+        }
+        __syncthreads();
+        double xa = sa, xb = sb;
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+            const double a0 = lds[i], a1 = lds[i + 1], a2 = lds[i + 2], a3 = lds[i + 3];
+            const double b0 = lds[n + i], b1 = lds[n + i + 1], b2 = lds[n + i + 2],
+                         b3 = lds[n + i + 3];
+            xa += a0;
+            xb += b0;
+            xa += a1;
+            xb += b1;
+            xa += a2;
+            xb += b2;
+            xa += a3;
+            xb += b3;
+        }
+        for (; i < n; i++) {
+            xa += lds[i];
+            xb += lds[n + i];
+        }
+        sa = xa;
+        sb = xb;
+    }
+    __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
+    // dense sample s.oidx of the lane's orbit into w_out (6, n_out, N)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[6]) const {
+        const int64_t so = a.n_out * a.N;
+        if (j < nb) FOR6 a.w_out[c * so + s.oidx * a.N + base + j] = v[c];
+    }
+};
+
+// Hairer's initial step guess (dop853.cpp:18-86) for a batched system: the
+// norms dnf, dny and der2 are sums over the system's components in its
+// state order (BatchLanes::sums).
+template <bool SCF, int LAY>
+__device__ double hinit_batch(const DevPot &P, const BatchLanes<SCF, LAY> &sys,
+                              const double (&y)[6], double posneg, const double (&f0)[6],
+                              double hmax, double rtol, double atol) {
+    double a[6], b[6];
+    FOR6 {
+        const double sk = atol + rtol * fabs(y[c]);
+        a[c] = f0[c] / sk;
+        b[c] = y[c] / sk;
+    }
+    double dnf = 0.0, dny = 0.0;
+    sys.sums(a, b, dnf, dny);
+    double h = ((dnf <= 1.0e-10) || (dny <= 1.0e-10)) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+    h = min_d(h, hmax);
+    h = sign_d(h, posneg);
+    double yy[6], f1[6];
+    FOR6 yy[c] = y[c] + h * f0[c];
+    sys.rhs(P, yy, f1);
+    FOR6 {
+        const double sk = atol + rtol * fabs(y[c]);
+        a[c] = (f1[c] - f0[c]) / sk;
+        b[c] = 0.0;
+    }
+    double der2 = 0.0, unused = 0.0;
+    sys.sums(a, b, der2, unused);
+    der2 = sqrt(der2) / h;
+    const double der12 = max_d(fabs(der2), sqrt(dnf));
+    const double h1 = (der12 <= 1.0e-15) ? max_d(1.0e-6, fabs(h) * 1.0e-3)
+                                         : pow(0.01 / der12, 1.0 / 8.0);
+    h = min_d(100.0 * fabs(h), min_d(h1, hmax));
+    return sign_d(h, posneg);
+}
+
 #define FORN _Pragma("unroll") for (int c = 0; c < NC; c++)
 
 // One step attempt (dopcor basic step, dop853.cpp:339-648) for the lane's NC
@@ -793,12 +891,15 @@ __device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, boo
 }
 
 // start the orbit at queue slot `slot`: state, first derivative and, for
-// h0 = 0, Hairer's initial step (dopcor's set-up, dop853.cpp:300-320)
+// h0 = 0, Hairer's initial step (dopcor's set-up, dop853.cpp:300-320).
+// Returns false for an order entry outside 0..N-1, which names no orbit: the
+// slot is skipped (an orbit that no slot names keeps the GAMD_STATUS_NOT_RUN
+// written before the launch).
 template <bool SCF, int LAY>
-__device__ __forceinline__ void load_orbit(const DevPot &P, const DopArgs &a, int64_t slot,
+__device__ __forceinline__ bool load_orbit(const DevPot &P, const DopArgs &a, int64_t slot,
                                            DopState &s, double (&y)[6], double (&k1)[6]) {
-    int64_t orb = a.order ? (int64_t)a.order[slot] : slot;
-    if (orb < 0 || orb >= a.N) orb = slot;  // malformed order: fall back
+    const int64_t orb = a.order ? (int64_t)a.order[slot] : slot;
+    if (orb < 0 || orb >= a.N) return false;
     s.orb = orb;
     FOR6 y[c] = a.w0[c * a.N + orb];
     s.x = a.t0[orb];
@@ -814,6 +915,7 @@ __device__ __forceinline__ void load_orbit(const DevPot &P, const DopArgs &a, in
     s.reject = s.iasti = s.nonsti = 0;
     orbit_rhs<SCF, LAY>(P, y, k1);
     if (s.h == 0.0) s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+    return true;
 }
 
 template <bool SCF, bool DENSE, int LAY>
@@ -843,10 +945,8 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
         if (wave * WIDE_ORBITS < a.n_tail) {
             const int64_t slot = wave * WIDE_ORBITS + lane;
-            if (lane < WIDE_ORBITS && slot < a.n_tail) {
-                load_orbit<SCF, LAY>(P, a, slot, s, y, k1);
-                active = true;
-            }
+            if (lane < WIDE_ORBITS && slot < a.n_tail)
+                active = load_orbit<SCF, LAY>(P, a, slot, s, y, k1);
             wide_tail<SCF, LAY>(P, a, active, s, y, k1);
             return;
         }
@@ -866,8 +966,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                 const unsigned long long slot = base + (unsigned long long)__popcll(below);
                 const unsigned long long qslot = slot + (unsigned long long)a.n_tail;
                 if (qslot < (unsigned long long)a.N) {
-                    load_orbit<SCF, LAY>(P, a, (int64_t)qslot, s, y, k1);
-                    active = true;
+                    active = load_orbit<SCF, LAY>(P, a, (int64_t)qslot, s, y, k1);
                 } else {
                     exhausted = true;
                 }
@@ -1071,6 +1170,66 @@ __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ 
     if (t == 0) finish_stats(a, s, code);
 }
 
+// dop853_integrate_hamiltonian with nbatch orbits per system (dop853.pyx:
+// 196-250): batch b, the orbits [b nbatch, min((b + 1) nbatch, N)), is one
+// (6, nb) system integrated by one workgroup (BatchLanes) from t0 to t1 with
+// initial step h0 (0: hinit over the batch) and dense output at t_out
+// (dop853_helper, :90-193).  The batch's status and step counts go to each of
+// its orbits.  Workgroups are independent: the hardware dispatcher hands
+// each free slot the next batch, which balances batches of uneven cost.
+template <bool SCF, bool DENSE, int LAY, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_dop853_batch(const DevPot *__restrict__ Pp, int nbatch,
+                                                      double t0, double t1, double h0, DopArgs a) {
+    extern __shared__ double lds[];
+    const DevPot &P = *Pp;
+    const int j = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * nbatch;
+    const int64_t left = a.N - base;
+    const int nb = left < (int64_t)nbatch ? (int)left : nbatch;
+    const BatchLanes<SCF, LAY> sys{j, nb, base, lds};
+    const int64_t orb = base + (j < nb ? j : 0);
+    double y[6], k1[6];
+    FOR6 y[c] = a.w0[c * a.N + orb];
+    DopState s;
+    s.orb = orb;
+    s.x = t0;
+    s.xend = t1;
+    s.h = h0;
+    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(t1 - t0);  // dop853.cpp:787-788, 312
+    s.posneg = sign_d(1.0, t1 - t0);
+    s.facold = 1.0e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
+    s.nfcn = 2;
+    s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = 0;
+    sys.rhs(P, y, k1);
+    if (s.h == 0.0) s.h = hinit_batch<SCF, LAY>(P, sys, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+    int code = 0;
+    // every attempt advances nstep and nmax bounds it; the step decisions are
+    // uniform over the workgroup, so its lanes leave together
+    while (code == 0) code = dop_attempt<SCF, DENSE, LAY>(P, a, sys, s, y, k1);
+    if (j < nb) {
+        FOR6 a.w_final[c * a.N + orb] = y[c];
+        finish_stats(a, s, code);
+    }
+}
+
+template <bool SCF, bool DENSE, int LAY>
+int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, const DopArgs &a,
+                 hipStream_t s) {
+    const int threads = (nbatch + 63) / 64 * 64;
+    const int64_t grid = (a.N + nbatch - 1) / nbatch;
+    const size_t lds = (size_t)12 * nbatch * sizeof(double);
+    if (threads <= 256)
+        hipLaunchKernelGGL((k_dop853_batch<SCF, DENSE, LAY, 256>), dim3((unsigned)grid),
+                           dim3(threads), lds, s, P, nbatch, t0, t1, h0, a);
+    else
+        hipLaunchKernelGGL((k_dop853_batch<SCF, DENSE, LAY, GAMD_DOP853_MAX_NBATCH>),
+                           dim3((unsigned)grid), dim3(threads), lds, s, P, nbatch, t0, t1, h0, a);
+    return launch_status();
+}
+
 // Systems of at most 4 bodies on one 16-lane row (two components per lane)
 // unless GAMD_DOP853_BODY_ROWS=0; =1 forces rows wherever they fit.  Same
 // results either way.
@@ -1083,6 +1242,13 @@ bool body_rows(int64_t systems, int nb) {
     // by its longest systems, whose attempts rows make shorter)
     return 16 * systems <= 2 * 64 * 4 * (int64_t)cu_count();
 }
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 template <bool SCF, bool DENSE, int LAY>
 int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
@@ -1128,7 +1294,11 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     hipStream_t s = as_stream(stream);
     unsigned long long *queue = nullptr;
     GAMD_HIP_CHECK(hipMallocAsync((void **)&queue, sizeof(unsigned long long), s));
-    GAMD_HIP_CHECK(hipMemsetAsync(queue, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, queue);  // a kernel, not a memset node
+    if (launch_status() != GAMD_OK) {
+        (void)hipFreeAsync(queue, s);
+        return GAMD_ERR_HIP;
+    }
     DopArgs a;
     a.w0 = w0;
     a.N = (int64_t)N;
@@ -1149,6 +1319,16 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     a.order = order;
     a.n_tail = n_tail;
     a.queue = queue;
+    if (order && status) {
+        // an orbit that the order does not name keeps this status
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, status,
+                           (int64_t)N, (int32_t)GAMD_STATUS_NOT_RUN);
+        const int rc = launch_status();
+        if (rc != GAMD_OK) {
+            (void)hipFreeAsync(queue, s);
+            return rc;
+        }
+    }
     // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end, =1 skips the
     // quad stage (same results every way)
     const char *wide_env = getenv("GAMD_DOP853_WIDE");
@@ -1290,4 +1470,47 @@ extern "C" int gamd_nbody_dop853(const gamd_potential *ext, const gamd_potential
                                      : k_dop853_nbody<false, false, LAYOUT_GENERIC, 64>));
     hipLaunchKernelGGL(kern, dim3(1), dim3(64), 0, s, ext->ddev, bt, t0, t1, h0, a);
     return launch_status();
+}
+
+extern "C" int gamd_dop853_batched(const gamd_potential *pot, const double *w0, size_t N,
+                                   double t0, double t1, double h0, double rtol, double atol,
+                                   int64_t nmax, int64_t nstiff, double hmax, int nbatch,
+                                   const double *t_out, size_t n_out, double *w_out,
+                                   double *w_final, int32_t *status, int64_t *stats,
+                                   void *stream) {
+    if (!pot || !w0 || !w_final || (w_out && (!t_out || n_out == 0))) return GAMD_ERR_INVALID;
+    if (nbatch < 1) return GAMD_ERR_INVALID;
+    if (nbatch > GAMD_DOP853_MAX_NBATCH) return GAMD_ERR_UNSUPPORTED;
+    if (nmax == 0) nmax = 1000000;                   // dop853.cpp:673-674
+    else if (nmax < 0) return GAMD_ERR_INVALID;      // :675-679
+    if (nstiff == 0) nstiff = 1000;                  // :691-692
+    else if (nstiff < 0) nstiff = 0;                 // :693-694 (never tested)
+    if (N == 0) return GAMD_OK;
+    if ((N + (size_t)nbatch - 1) / (size_t)nbatch > 0x7fffffffu) return GAMD_ERR_UNSUPPORTED;
+    DopArgs a;
+    memset(&a, 0, sizeof(a));
+    a.w0 = w0;
+    a.N = (int64_t)N;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.nmax = nmax;
+    a.nstiff = nstiff;
+    a.hmax = hmax;
+    a.t_out = t_out;
+    a.n_out = w_out ? (int64_t)n_out : 0;
+    a.w_out = w_out;
+    a.w_final = w_final;
+    a.status = status;
+    a.stats = stats;
+    a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
+    const bool scf = pot->dev.has_scf;
+    const bool mw = pot->dev.layout == LAYOUT_MW;
+    hipStream_t s = as_stream(stream);
+    if (w_out)
+        return scf  ? launch_batch<true, true, LAYOUT_GENERIC>(pot->ddev, nbatch, t0, t1, h0, a, s)
+               : mw ? launch_batch<false, true, LAYOUT_MW>(pot->ddev, nbatch, t0, t1, h0, a, s)
+                    : launch_batch<false, true, LAYOUT_GENERIC>(pot->ddev, nbatch, t0, t1, h0, a, s);
+    return scf  ? launch_batch<true, false, LAYOUT_GENERIC>(pot->ddev, nbatch, t0, t1, h0, a, s)
+           : mw ? launch_batch<false, false, LAYOUT_MW>(pot->ddev, nbatch, t0, t1, h0, a, s)
+                : launch_batch<false, false, LAYOUT_GENERIC>(pot->ddev, nbatch, t0, t1, h0, a, s);
 }

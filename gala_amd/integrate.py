@@ -14,6 +14,7 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
+           "dop853_batched",
            "pilot_order", "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
 
 
@@ -78,16 +79,16 @@ def parse_time_specification(units=None, dt=None, n_steps=None, t1=None, t2=None
         return parse_time_specification(units, dt=np.ones(n_steps + 1) * dt, t1=t1)
     if dt is not None and t1 is not None and t2 is not None:
         if t2 < t1 and dt < 0:
-            t_i, times, ii = t1, [], 0
-            while (t_i > t2) and (ii < 1e6):
+            t_i, times = t1, []
+            while t_i > t2:
                 times.append(t_i)
                 t_i += dt
             if times[-1] != t2:
                 times.append(t2)
             return np.array(times, dtype=np.float64)
         if t2 > t1 and dt > 0:
-            t_i, times, ii = t1, [], 0
-            while (t_i < t2) and (ii < 1e6):
+            t_i, times = t1, []
+            while t_i < t2:
                 times.append(t_i)
                 t_i += dt
             return np.array(times, dtype=np.float64)
@@ -110,13 +111,13 @@ def leapfrog(potential, w0, dt, n_steps, save_stride=1, save=True, n_steps_per_o
     """Leapfrog on device.  w0: (6, N) float64 device tensor.
     Returns (w_out (6, n_steps//save_stride + 1, N) or None, w_final (6, N))."""
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     N = w0.shape[1]
     wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, N)) if save else None
     wfin = dev.empty((6, N))
     nsp = None
     if n_steps_per_orbit is not None:
-        nsp = dev.to_dev(n_steps_per_orbit, dtype=dev.torch().int64)
+        nsp = dev.per_orbit(n_steps_per_orbit, N, "n_steps_per_orbit", dtype=dev.torch().int64)
         n_steps = int(nsp.max().item()) if N else 0
     _lib.check(lib.gamd_leapfrog(potential.c_handle, w0.data_ptr(), N, float(t0), float(dt),
                                  int(n_steps), dev.ptr(nsp), int(save_stride), dev.ptr(wout),
@@ -128,7 +129,7 @@ def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
     """Ruth4 on device (ruth4.pyx:24-106).  w0: (6, N) float64.
     Returns (w_out (6, n_steps//save_stride + 1, N) or None, w_final (6, N))."""
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     N = w0.shape[1]
     wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, N)) if save else None
     wfin = dev.empty((6, N))
@@ -153,22 +154,13 @@ def pilot_order(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, 
     Eccentric orbits through the nucleus cost many more steps per orbital
     period than their dynamical time suggests; the pilot measures it."""
     t = dev.require_gpu()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     N = w0.shape[1]
-    t0d, t1d = _per_orbit(t0, N), _per_orbit(t1, N)
+    t0d, t1d = dev.per_orbit(t0, N, "t0"), dev.per_orbit(t1, N, "t1")
     rp = dop853(potential, w0, t0d, t0d + (t1d - t0d) * fraction, h0, rtol=rtol, atol=atol,
                 nmax=nmax, nstiff=nstiff, hmax=hmax, block=block)
     est = rp["stats"][:, 0].to(t.float64)
     return t.argsort(est, descending=True, stable=True).to(t.int32)
-
-
-def _per_orbit(v, N):
-    t = dev.torch()
-    if v is None:
-        return None
-    if isinstance(v, t.Tensor):
-        return dev.to_dev(v.expand(N) if v.dim() == 0 else v)
-    return dev.to_dev(np.broadcast_to(np.asarray(v, dtype=np.float64), (N,)))
 
 
 def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0, hmax=0.0,
@@ -178,16 +170,21 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
     pilot_order's estimate, computed in the same call); n_tail: the first
     n_tail orbits of that order run on dedicated 16-lane rows from the start.
     Returns dict(w_final, status, stats[, w_out])."""
+    N = dev.check_w0(w0)
+    for v, name in ((t0, "t0"), (t1, "t1"), (h0, "h0")):
+        dev.check_per_orbit(v, N, name)
+    if order is not None and not isinstance(order, str):
+        dev.check_permutation(order, N)
     t = dev.require_gpu()
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
-    N = w0.shape[1]
+    w0 = dev.phase_space(w0)
     if isinstance(order, str):
         if order != "pilot":
             raise ValueError(f"unknown order {order!r}")
         order = pilot_order(potential, w0, t0, t1, h0, rtol=rtol, atol=atol, nmax=nmax,
                             nstiff=nstiff, hmax=hmax, block=block)
-    t0d, t1d, h0d = _per_orbit(t0, N), _per_orbit(t1, N), _per_orbit(h0, N)
+    t0d, t1d = dev.per_orbit(t0, N, "t0"), dev.per_orbit(t1, N, "t1")
+    h0d = dev.per_orbit(h0, N, "h0")
     wfin = dev.empty((6, N))
     status = dev.empty((N,), dtype=t.int32)
     stats = dev.empty((N, 4), dtype=t.int64)
@@ -195,7 +192,7 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
     if t_out is not None:
         tout = dev.to_dev(t_out)
         wout = dev.empty((6, tout.shape[0], N))
-    ordd = dev.to_dev(order, dtype=t.int32) if order is not None else None
+    ordd = dev.to_dev(order, dtype=t.int32) if order is not None else None  # checked above
     _lib.check(lib.gamd_dop853(potential.c_handle, w0.data_ptr(), N, t0d.data_ptr(),
                                t1d.data_ptr(), dev.ptr(h0d), float(rtol), float(atol), int(nmax),
                                int(nstiff), float(hmax), dev.ptr(tout),
@@ -203,6 +200,46 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
                                wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
                                dev.ptr(ordd), int(n_tail), int(block), dev.stream_ptr()),
                "gamd_dop853")
+    out = {"w_final": wfin, "status": status, "stats": stats}
+    if wout is not None:
+        out["w_out"] = wout
+    return out
+
+
+MAX_NBATCH = 1024  # GAMD_DOP853_MAX_NBATCH: orbits of one batched system
+
+
+def dop853_batched(potential, w0, t0, t1, h0, nbatch=100, rtol=1e-10, atol=1e-10, nmax=0,
+                   nstiff=0, hmax=0.0, t_out=None):
+    """DOP853 with gala's default batching (dop853_integrate_hamiltonian,
+    dop853.pyx:196-250): orbits [i, i + nbatch) of w0 (6, N) form one
+    (6, nb) system with one shared adaptive step, integrated from t0 to t1
+    (scalars) with initial step h0 (gala passes t[1] - t[0]); dense output at
+    t_out.  1 <= nbatch <= MAX_NBATCH.  Returns dict(w_final, status, stats
+    [, w_out]) with each orbit's batch's status and step counts."""
+    N = dev.check_w0(w0)
+    nbatch = int(nbatch)
+    if nbatch < 1:
+        raise ValueError(f"nbatch must be at least 1, got {nbatch}")
+    if nbatch > MAX_NBATCH:
+        raise NotImplementedError(f"nbatch={nbatch}: one batched system holds at most "
+                                  f"{MAX_NBATCH} orbits on the GPU (one workgroup)")
+    t = dev.require_gpu()
+    lib = _lib.load()
+    w0 = dev.phase_space(w0)
+    wfin = dev.empty((6, N))
+    status = dev.empty((N,), dtype=t.int32)
+    stats = dev.empty((N, 4), dtype=t.int64)
+    wout = tout = None
+    if t_out is not None:
+        tout = dev.to_dev(t_out)
+        wout = dev.empty((6, tout.shape[0], N))
+    _lib.check(lib.gamd_dop853_batched(potential.c_handle, w0.data_ptr(), N, float(t0),
+                                       float(t1), float(h0), float(rtol), float(atol), int(nmax),
+                                       int(nstiff), float(hmax), nbatch, dev.ptr(tout),
+                                       0 if tout is None else tout.shape[0], dev.ptr(wout),
+                                       wfin.data_ptr(), status.data_ptr(), stats.data_ptr(),
+                                       dev.stream_ptr()), "gamd_dop853_batched")
     out = {"w_final": wfin, "status": status, "stats": stats}
     if wout is not None:
         out["w_out"] = wout
@@ -222,7 +259,7 @@ def nbody_leapfrog(potential, body_potentials, w0, dt, n_steps, save_stride=1, s
     centred on its body; bodies K.. of w0 (6, n) are test particles.
     Returns (w_out (6, n_steps//save_stride + 1, n) or None, w_final (6, n))."""
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     n = w0.shape[1]
     arr, K = _bodies(body_potentials)
     wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, n)) if save else None
@@ -237,7 +274,7 @@ def nbody_ruth4(potential, body_potentials, w0, dt, n_steps, save_stride=1, save
     """ruth4_integrate_nbody on device (ruth4.pyx:138-261); arguments and
     results as nbody_leapfrog."""
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     n = w0.shape[1]
     arr, K = _bodies(body_potentials)
     wout = dev.empty((6, int(n_steps) // int(save_stride) + 1, n)) if save else None
@@ -255,7 +292,7 @@ def nbody_dop853(potential, body_potentials, w0, t0, t1, h0, rtol=1e-10, atol=1e
     Returns dict(w_final (6, n), status (1,), stats (1, 4)[, w_out (6, n_out, n)])."""
     t = dev.require_gpu()
     lib = _lib.load()
-    w0 = dev.to_dev(w0)
+    w0 = dev.phase_space(w0)
     n = w0.shape[1]
     arr, K = _bodies(body_potentials)
     wfin = dev.empty((6, n))
@@ -298,11 +335,14 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
     """The dispatch of chamiltonian.pyx:274-380 onto the HIP kernels.
 
     Leapfrog: dt = t[1]-t[0], len(t)-1 steps (leapfrog.pyx:78, 106).
-    DOP853: every orbit is its own adaptive system from t[0] to t[-1] with
-    h0 = t[1]-t[0] (gala with nbatch=1, dop853.pyx:171, 228-245); with
-    save_all the dense output samples every t (dop853.cpp:584-612).
-    Integrator_kwargs: atol, rtol (default 1e-10), nmax, err_if_fail
-    (default 1, raise on a negative status), nstiff, hmax.
+    DOP853: orbits are integrated from t[0] to t[-1] with h0 = t[1]-t[0]
+    (dop853.pyx:171) in batches of nbatch (default 100, chamiltonian.pyx:348)
+    orbits sharing one adaptive step (dop853.pyx:228-245); nbatch=1 makes
+    every orbit its own system.  With save_all the dense output samples every
+    t (dop853.cpp:584-612).  Integrator_kwargs: atol, rtol (default 1e-10),
+    nmax, err_if_fail (default 1, raise on a negative status), nbatch, and,
+    beyond gala's keys, nstiff and dt_max.  log_output is accepted and
+    ignored (the kernels print nothing).
     """
     from .dynamics import Orbit
     Integrator_kwargs = dict(Integrator_kwargs or {})
@@ -326,13 +366,19 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
                            save=bool(save_all), t0=t[0])
         w = dev.to_host(wout) if save_all else dev.to_host(wfin)
     elif Integrator is DOPRI853Integrator:
-        res = dop853(pot, arr, t[0], t[-1], t[1] - t[0],
-                     rtol=Integrator_kwargs.get("rtol", 1e-10),
-                     atol=Integrator_kwargs.get("atol", 1e-10),
-                     nmax=Integrator_kwargs.get("nmax", 0),
-                     nstiff=Integrator_kwargs.get("nstiff", 0),
-                     hmax=Integrator_kwargs.get("dt_max", 0.0),
-                     t_out=t if save_all else None)
+        if len(t) < 2:
+            raise ValueError("need at least two times")
+        kw = dict(rtol=Integrator_kwargs.get("rtol", 1e-10),
+                  atol=Integrator_kwargs.get("atol", 1e-10),
+                  nmax=Integrator_kwargs.get("nmax", 0),
+                  nstiff=Integrator_kwargs.get("nstiff", 0),
+                  hmax=Integrator_kwargs.get("dt_max", 0.0),
+                  t_out=t if save_all else None)
+        nbatch = int(Integrator_kwargs.get("nbatch", 100))  # chamiltonian.pyx:348
+        if nbatch == 1:
+            res = dop853(pot, arr, t[0], t[-1], t[1] - t[0], **kw)
+        else:
+            res = dop853_batched(pot, arr, t[0], t[-1], t[1] - t[0], nbatch=nbatch, **kw)
         status = dev.to_host(res["status"])
         if int(Integrator_kwargs.get("err_if_fail", 1)) == 1 and np.any(status < 0):
             bad = int(status[status < 0][0])

@@ -33,6 +33,10 @@ enum gamd_status {
     GAMD_ERR_UNSUPPORTED = -4
 };
 
+/* status[] of an orbit that an integration never started (gamd_dop853 with
+ * an order that does not name it); gala's own codes are 1, -2, -3, -4 */
+#define GAMD_STATUS_NOT_RUN 0
+
 /* ---- potentials -------------------------------------------------------- */
 enum gamd_component_type {
     GAMD_NULL = 0,           /* NullWrapper, cybuiltin.pyx:362-372 */
@@ -131,7 +135,11 @@ int gamd_ruth4(const gamd_potential *pot, const double *w0, size_t N, double t0,
  * (N,4) = nstep, naccpt, nrejct, nfcn (nullable) are per orbit.
  * order (device, nullable): the order in which the persistent kernel's work
  * queue hands orbits to lanes (a permutation of 0..N-1; longest first keeps
- * the tail short).  n_tail >= 0: the first n_tail orbits of that order are
+ * the tail short).  With an order, status is first set to
+ * GAMD_STATUS_NOT_RUN for every orbit; entries outside 0..N-1 are skipped,
+ * so an orbit that the order does not name keeps that status (the device
+ * array is not read back to check it; gala_amd.integrate.dop853 checks that
+ * a caller's order is a permutation before the call).  n_tail >= 0: the first n_tail orbits of that order are
  * integrated from the start by dedicated waves, four orbits per wave, each
  * spread over 16 lanes (capped at half the grid; ignored with dense output);
  * results are identical for every n_tail.  block: threads per workgroup,
@@ -143,6 +151,28 @@ int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const dou
                 int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
                 double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
                 int64_t n_tail, int block, void *stream);
+
+#define GAMD_DOP853_MAX_NBATCH 1024  /* orbits of one batched system (one workgroup) */
+
+/* Replaces dop853_integrate_hamiltonian with gala's default batching
+ * (dop853.pyx:196-250; Hamiltonian.integrate_orbit passes nbatch=100,
+ * chamiltonian.pyx:348): orbits [b nbatch, min((b+1) nbatch, N)) form one
+ * (6, nb) system with ONE shared adaptive step (dop853_helper, :90-193),
+ * integrated from t0 to t1 (host scalars) with initial step h0 (gala passes
+ * t[1]-t[0], dop853.pyx:171; 0 = hinit over the batch) and hmax = |t1-t0|
+ * when hmax <= 0.  The error norm and the stiffness test sum over the
+ * system's 6 nb components in the reference's order (dop853.cpp:416-444,
+ * 463-486), so one orbit's error shapes the whole batch's steps as in gala.
+ * nmax / nstiff defaults as gamd_dop853.  Dense output at t_out (device,
+ * n_out) into w_out (6, n_out, N) when w_out != NULL.  w_final (6, N);
+ * status (N) and stats (N, 4) (nullable) hold each orbit's batch's idid and
+ * nstep, naccpt, nrejct, nfcn.  1 <= nbatch <= GAMD_DOP853_MAX_NBATCH
+ * (GAMD_ERR_UNSUPPORTED above). */
+int gamd_dop853_batched(const gamd_potential *pot, const double *w0, size_t N, double t0,
+                        double t1, double h0, double rtol, double atol, int64_t nmax,
+                        int64_t nstiff, double hmax, int nbatch, const double *t_out,
+                        size_t n_out, double *w_out, double *w_final, int32_t *status,
+                        int64_t *stats, void *stream);
 
 /* ---- N-body systems ------------------------------------------------------ */
 #define GAMD_MAX_BODIES 64   /* bodies of one DOP853 N-body system (one wave) */

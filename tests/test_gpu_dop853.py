@@ -63,7 +63,7 @@ def test_dense_output_parity():
     t = np.linspace(0.0, 600.0, 301)
     ref, st, _ = O.dop853(orc, w0, t, atol=1e-10, rtol=1e-10, nbatch=1, save_all=True)
     orbit = Hamiltonian(mw_product()).integrate_orbit(
-        w0, t=t, Integrator="dopri853", Integrator_kwargs=dict(atol=1e-10, rtol=1e-10))
+        w0, t=t, Integrator="dopri853", Integrator_kwargs=dict(atol=1e-10, rtol=1e-10, nbatch=1))
     w = orbit.w()
     assert w.shape == (6, 301, n)
     assert np.max(state_rel_err(w, ref)) < 1e-8
@@ -84,7 +84,7 @@ def test_status_codes_and_errors():
     assert st_o[0] == -2
     with pytest.raises(RuntimeError):
         Hamiltonian(pot).integrate_orbit(w0[:, 0], dt=1.0, n_steps=5000, Integrator="dop853",
-                                         Integrator_kwargs=dict(nmax=5))
+                                         Integrator_kwargs=dict(nmax=5, nbatch=1))
     # h0 = 0 -> Hairer's initial step guess (dop853.cpp:18-86), still correct
     res = dop853(pot, w0, 0.0, 500.0, 0.0, rtol=1e-10, atol=1e-10)
     ref = O.dop853_orbits(mw_oracle(), w0, 0.0, 500.0, 1.0, atol=1e-10, rtol=1e-10)[0]

@@ -63,6 +63,47 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     assert lib.gamd_potential_destroy(None) == 0
 
 
+def test_batched_dop853_arguments_rejected():
+    """gamd_dop853_batched rejects null pointers and a batch wider than one
+    workgroup before any HIP call."""
+    from gala_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libgala_amd.so not built")
+    lib = _lib.load()
+    args = [0.0, 1.0, 1.0, 1e-10, 1e-10, 0, 0, 0.0]
+    assert lib.gamd_dop853_batched(None, None, 1, *args, 100, None, 0, None, None, None, None,
+                                   None) == -1
+    fake = ctypes.c_void_p(8)  # never dereferenced: the nbatch checks come first
+    assert lib.gamd_dop853_batched(fake, fake, 1, *args, 0, None, 0, None, fake, None, None,
+                                   None) == -1
+    assert lib.gamd_dop853_batched(fake, fake, 1, *args, 1025, None, 0, None, fake, None, None,
+                                   None) == -4
+
+
+def test_device_argument_checks_run_on_the_host():
+    """Shapes, per-orbit lengths and queue orders are checked before any
+    device work: a (5, N) w0, a t1 of the wrong length or an order that is not
+    a permutation raise ValueError (they would otherwise be read past their
+    ends or leave orbits unvisited)."""
+    import torch
+    from gala_amd import _device as dev
+    assert dev.check_w0(np.zeros((6, 7))) == 7
+    assert dev.check_w0(torch.zeros(6, 3, dtype=torch.float32)) == 3
+    for bad in (np.zeros((5, 7)), np.zeros(6), np.zeros((6, 2, 2))):
+        with pytest.raises(ValueError):
+            dev.check_w0(bad)
+    dev.check_per_orbit(1.0, 7, "t1")
+    dev.check_per_orbit(np.ones(7), 7, "t1")
+    with pytest.raises(ValueError):
+        dev.check_per_orbit(np.ones(6), 7, "t1")
+    dev.check_permutation(np.array([2, 0, 1]), 3)
+    dev.check_permutation(torch.tensor([1, 0], dtype=torch.int32), 2)
+    for bad in (np.array([0, 1]), np.array([0, 0, 1]), np.array([0, 1, 3]),
+                np.array([-1, 0, 1]), torch.tensor([0, 2, 2])):
+        with pytest.raises(ValueError):
+            dev.check_permutation(bad, 3)
+
+
 def test_parse_time_specification():
     from gala_amd.integrate import parse_time_specification as p
     t = p(dt=1.0, n_steps=10)
