@@ -1274,12 +1274,12 @@ int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
 
 }  // namespace
 
-extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
-                           const double *t1, const double *h0, double rtol, double atol,
-                           int64_t nmax, int64_t nstiff, double hmax, const double *t_out,
-                           size_t n_out, double *w_out, double *w_final, int32_t *status,
-                           int64_t *stats, const int32_t *order, int64_t n_tail, int block,
-                           void *stream) {
+namespace {
+
+// gamd_dop853's argument checks and gala's defaults (dop853.cpp:673-694)
+int dop853_check(const gamd_potential *pot, const double *w0, const double *t0, const double *t1,
+                 const double *w_final, const double *w_out, const double *t_out, size_t n_out,
+                 int64_t n_tail, int &block, int64_t &nmax, int64_t &nstiff) {
     if (!pot || !w0 || !t0 || !t1 || !w_final || (w_out && (!t_out || n_out == 0)))
         return GAMD_ERR_INVALID;
     if (n_tail < 0) return GAMD_ERR_INVALID;
@@ -1290,15 +1290,20 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     else if (nmax < 0) return GAMD_ERR_INVALID;      // :675-679
     if (nstiff == 0) nstiff = 1000;                  // :691-692
     else if (nstiff < 0) nstiff = 0;                 // :693-694 (never tested)
-    if (N == 0) return GAMD_OK;
-    hipStream_t s = as_stream(stream);
-    unsigned long long *queue = nullptr;
-    GAMD_HIP_CHECK(hipMallocAsync((void **)&queue, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, queue);  // a kernel, not a memset node
-    if (launch_status() != GAMD_OK) {
-        (void)hipFreeAsync(queue, s);
-        return GAMD_ERR_HIP;
-    }
+    return GAMD_OK;
+}
+
+// The launches of gamd_dop853 after its checks, with the work-queue counter
+// in `queue` (device memory the caller provides; cleared here by a kernel,
+// so that the sequence holds no memset node).
+int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
+               const double *t1, const double *h0, double rtol, double atol, int64_t nmax,
+               int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
+               double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
+               int64_t n_tail, int block, unsigned long long *queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, queue);
+    int rc = launch_status();
+    if (rc != GAMD_OK) return rc;
     DopArgs a;
     a.w0 = w0;
     a.N = (int64_t)N;
@@ -1323,29 +1328,42 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
         // an orbit that the order does not name keeps this status
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, status,
                            (int64_t)N, (int32_t)GAMD_STATUS_NOT_RUN);
-        const int rc = launch_status();
-        if (rc != GAMD_OK) {
-            (void)hipFreeAsync(queue, s);
-            return rc;
-        }
+        rc = launch_status();
+        if (rc != GAMD_OK) return rc;
     }
     // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end, =1 skips the
     // quad stage (same results every way)
     const char *wide_env = getenv("GAMD_DOP853_WIDE");
     a.wide = !wide_env ? 2 : (wide_env[0] == '0' ? 0 : (wide_env[0] == '1' ? 1 : 2));
     a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
-    int rc;
     const bool scf = pot->dev.has_scf;
     const bool mw = pot->dev.layout == LAYOUT_MW;
     if (w_out)
-        rc = scf  ? launch<true, true, LAYOUT_GENERIC>(pot->ddev, a, block, s)
-             : mw ? launch<false, true, LAYOUT_MW>(pot->ddev, a, block, s)
-                  : launch<false, true, LAYOUT_GENERIC>(pot->ddev, a, block, s);
-    else
-        rc = scf  ? launch<true, false, LAYOUT_GENERIC>(pot->ddev, a, block, s)
-             : mw ? launch<false, false, LAYOUT_MW>(pot->ddev, a, block, s)
-                  : launch<false, false, LAYOUT_GENERIC>(pot->ddev, a, block, s);
-    hipError_t fe = hipFreeAsync(queue, s);
+        return scf  ? launch<true, true, LAYOUT_GENERIC>(pot->ddev, a, block, s)
+               : mw ? launch<false, true, LAYOUT_MW>(pot->ddev, a, block, s)
+                    : launch<false, true, LAYOUT_GENERIC>(pot->ddev, a, block, s);
+    return scf  ? launch<true, false, LAYOUT_GENERIC>(pot->ddev, a, block, s)
+           : mw ? launch<false, false, LAYOUT_MW>(pot->ddev, a, block, s)
+                : launch<false, false, LAYOUT_GENERIC>(pot->ddev, a, block, s);
+}
+
+}  // namespace
+
+extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
+                           const double *t1, const double *h0, double rtol, double atol,
+                           int64_t nmax, int64_t nstiff, double hmax, const double *t_out,
+                           size_t n_out, double *w_out, double *w_final, int32_t *status,
+                           int64_t *stats, const int32_t *order, int64_t n_tail, int block,
+                           void *stream) {
+    int rc = dop853_check(pot, w0, t0, t1, w_final, w_out, t_out, n_out, n_tail, block, nmax,
+                          nstiff);
+    if (rc != GAMD_OK || N == 0) return rc;
+    hipStream_t s = as_stream(stream);
+    unsigned long long *queue = nullptr;
+    GAMD_HIP_CHECK(hipMallocAsync((void **)&queue, sizeof(unsigned long long), s));
+    rc = dop853_run(pot, w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, w_out,
+                    w_final, status, stats, order, n_tail, block, queue, s);
+    const hipError_t fe = hipFreeAsync(queue, s);
     if (rc == GAMD_OK && fe != hipSuccess) rc = GAMD_ERR_HIP;
     return rc;
 }
