@@ -623,6 +623,41 @@ typedef struct {
     long nstep, naccpt, nrejct, nfcn;
 } orc_dopstats;
 
+/* hinit, dop853.cpp:18-86 (scalar tolerances, iord = 8): Hairer's initial
+ * step guess from the norms of f0 and y, one explicit Euler step and the
+ * resulting estimate of the second derivative.  f1 and yy1 are work arrays. */
+static double orc_hinit(const orc_sys *S, double x, const double *y, double posneg,
+                        const double *f0, double *f1, double *yy1, double hmax, double rtoli,
+                        double atoli) {
+    const int n = S->n;
+    double dnf = 0.0, dny = 0.0;
+    (void)x;
+    for (int i = 0; i < n; i++) {
+        double sk = atoli + rtoli * fabs(y[i]);
+        double sqr = f0[i] / sk;
+        dnf += sqr * sqr;
+        sqr = y[i] / sk;
+        dny += sqr * sqr;
+    }
+    double h = ((dnf <= 1.0e-10) || (dny <= 1.0e-10)) ? 1.0e-6 : sqrt(dny / dnf) * 0.01;
+    h = min_d(h, hmax);
+    h = sign_d(h, posneg);
+    for (int i = 0; i < n; i++) yy1[i] = y[i] + h * f0[i];
+    S->f(S, yy1, f1);
+    double der2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        double sk = atoli + rtoli * fabs(y[i]);
+        double sqr = (f1[i] - f0[i]) / sk;
+        der2 += sqr * sqr;
+    }
+    der2 = sqrt(der2) / h;
+    double der12 = max_d(fabs(der2), sqrt(dnf));
+    double h1 = (der12 <= 1.0e-15) ? max_d(1.0e-6, fabs(h) * 1.0e-3)
+                                   : pow(0.01 / der12, 1.0 / 8.0);
+    h = min_d(100.0 * fabs(h), min_d(h1, hmax));
+    return sign_d(h, posneg);
+}
+
 /* dopcor, :89-650, specialised to scalar tolerances, no solout, beta=0,
  * meth=1, with gala's dense-output fill at out_t (:584-612).  y (n=6 nb)
  * is advanced in place.  Returns the idid code. */
@@ -649,6 +684,7 @@ static int orc_dopcor(const orc_sys *S, double x, double *y, double xend, double
 
     S->f(S, y, k1);
     hmax = fabs(hmax);
+    if (h == 0.0) h = orc_hinit(S, x, y, posneg, k1, k2, k3, hmax, rtoli, atoli); /* :315-317 */
     st->nfcn += 2; /* :319 (counts the hinit evaluation slot too) */
 
     while (1) {

@@ -47,6 +47,23 @@ def test_dop853_bitwise(nbatch, save_all):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("nbatch", [1, 4])
+def test_dop853_hinit_bitwise(nbatch):
+    """h0 = t[1] - t[0] = 0: dopcor starts from Hairer's initial step guess
+    (hinit, dop853.cpp:18-86, called at :315-317), over the whole batch."""
+    if O.ref_lib() is None:
+        pytest.skip("gala's sources are not present")
+    comps = O.milky_way_like()
+    rng = np.random.default_rng(17)
+    n = 8
+    w0 = np.vstack([rng.uniform(2, 25, (3, n)), rng.normal(0, 0.15, (3, n))])
+    for t in (np.array([0.0, 0.0, 700.0]), np.array([0.0, 0.0, -700.0])):
+        a, st_a, stats = O.dop853(comps, w0, t, nbatch=nbatch, save_all=False)
+        b, st_b = O.dop853(comps, w0, t, nbatch=nbatch, save_all=False, ref=True)
+        assert np.all(st_a == 1) and np.array_equal(st_a, st_b)
+        assert np.array_equal(a, b)
+
+
 def test_dop853_backward_and_status_codes():
     comps = O.milky_way_like()
     w0 = np.array([[8.0], [0.1], [0.2], [0.01], [0.2], [0.02]])
