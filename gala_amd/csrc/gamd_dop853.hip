@@ -27,6 +27,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "dop853_tableau.h"
@@ -1366,6 +1367,62 @@ extern "C" int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N
     const hipError_t fe = hipFreeAsync(queue, s);
     if (rc == GAMD_OK && fe != hipSuccess) rc = GAMD_ERR_HIP;
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// gamd_dop853_noalloc: gamd_dop853 with its work-queue counter in `scratch`,
+// a fixed device array of this library, instead of a per-call stream-ordered
+// allocation.  Each GPU holds its own copy of the array, at its own address,
+// from the time the library's code object is loaded there until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns with that GPU current (the array's name in host code is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned long long scratch[GAMD_DOP853_SCRATCH_BYTES / 8];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned long long *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The current GPU's copy of `scratch`, looked up once per GPU.
+int scratch_address(unsigned long long **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return GAMD_ERR_HIP;
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return GAMD_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "gala_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return GAMD_ERR_HIP;
+        }
+        scratch_dev[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = scratch_dev[dev];
+    return GAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, size_t N,
+                                   const double *t0, const double *t1, const double *h0,
+                                   double rtol, double atol, int64_t nmax, int64_t nstiff,
+                                   double hmax, const double *t_out, size_t n_out, double *w_out,
+                                   double *w_final, int32_t *status, int64_t *stats,
+                                   const int32_t *order, int64_t n_tail, int block,
+                                   void *stream) {
+    int rc = dop853_check(pot, w0, t0, t1, w_final, w_out, t_out, n_out, n_tail, block, nmax,
+                          nstiff);
+    if (rc != GAMD_OK || N == 0) return rc;
+    unsigned long long *queue = nullptr;
+    rc = scratch_address(&queue);
+    if (rc != GAMD_OK) return rc;
+    // the counter is the array's first word; the kernels read no other
+    return dop853_run(pot, w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, w_out,
+                      w_final, status, stats, order, n_tail, block, queue, as_stream(stream));
 }
 
 extern "C" int gamd_mockstream_dop853(const gamd_potential *ext,

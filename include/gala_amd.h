@@ -152,6 +152,26 @@ int gamd_dop853(const gamd_potential *pot, const double *w0, size_t N, const dou
                 double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
                 int64_t n_tail, int block, void *stream);
 
+/* gamd_dop853 for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Arguments, results and errors are
+ * gamd_dop853's; the work-queue counter, which gamd_dop853 allocates per
+ * call, lives instead in a fixed device array of the library
+ * (GAMD_DOP853_SCRATCH_BYTES per GPU, held from the library's load on that
+ * GPU until the process ends).  No input size is limited by it: the counter
+ * does not grow with N.  The call makes no hipMalloc, hipFree or other
+ * allocating call; it clears the counter with a kernel (no memset node).
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one
+ * GPU (concurrent streams, or threads) race: serialise them.  The GPU is the
+ * current device of the calling thread. */
+#define GAMD_DOP853_SCRATCH_BYTES 256
+int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, size_t N,
+                        const double *t0, const double *t1, const double *h0, double rtol,
+                        double atol, int64_t nmax, int64_t nstiff, double hmax,
+                        const double *t_out, size_t n_out, double *w_out, double *w_final,
+                        int32_t *status, int64_t *stats, const int32_t *order, int64_t n_tail,
+                        int block, void *stream);
+
 #define GAMD_DOP853_MAX_NBATCH 1024  /* orbits of one batched system (one workgroup) */
 
 /* Replaces dop853_integrate_hamiltonian with gala's default batching
