@@ -48,6 +48,28 @@ def traffic_from_profiles(name):
         return None
 
 
+def host_cpu():
+    """Model name, logical CPUs this process may use, and SMT state of the
+    host whose cores run the CPU baseline."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus_available": len(os.sched_getaffinity(0)),
+            "smt_active": smt}
+
+
 def cpu_baseline_dop853(w0, T, stride):
     from oracle import oracle as O
     orc = O.milky_way_like()
@@ -60,7 +82,8 @@ def cpu_baseline_dop853(w0, T, stride):
             "cores": int(O.num_threads(fast=True)), "kind": "port",
             "sample": f"every {stride}th orbit of the same workload ({len(sel)} orbits, "
                       f"{int(stats[:, 1].sum())} accepted steps, {dt:.1f} s), oracle/gala_oracle.c "
-                      f"built -Ofast with OpenMP"}
+                      f"built -Ofast with OpenMP",
+            "host": host_cpu()}
 
 
 def bench_dop853(args):
@@ -130,8 +153,10 @@ def bench_dop853(args):
     nfcn = int(stats[:, 3].sum())
     ms_per_step = wall * 1000.0 / args.steps
     value = naccpt / (ms_per_step / 1000.0)
+    codes, counts = np.unique(status, return_counts=True)
+    status_hist = {str(int(c)): int(k) for c, k in zip(codes, counts)}
     log(f"{args.steps} passes: {ms_per_step:.1f} ms/pass, pilot {pilot_ms:.1f} ms + kernel {kernel_ms:.1f} ms; "
-        f"{naccpt} accepted, {nrejct} rejected, status ok {np.mean(status == 1):.6f}")
+        f"{naccpt} accepted, {nrejct} rejected, status counts {status_hist}")
     flops = roofline.dop853_flops(naccpt, nrejct)
     achieved = flops / (kernel_ms / 1000.0) / 1e12
     traffic = traffic_from_profiles("dop853_composite")
@@ -152,8 +177,11 @@ def bench_dop853(args):
                    "autotune_ms": None if tuned is None else tuned["trials_ms"]},
         "accepted_steps": naccpt, "rejected_steps": nrejct,
         "max_attempts_one_orbit": int(stats[:, 0].max()),
+        # gala's codes: 1 success, -4 stiffness detected (dop853.cpp:474-481
+        # returns on the first detection; the oracle gives the same orbits -4)
+        "status_counts": status_hist,
         "fevals_per_s": nfcn / (ms_per_step / 1000.0),
-        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": achieved,
+        "roofline": {"bound": "fp64-valu", "pipe": "fp64 (VALU)", "achieved": achieved,
                      "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / roofline.PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "k_dop853<false,false> (the full launch)",
@@ -216,10 +244,23 @@ def bench_leapfrog(args):
         c0 = time.perf_counter()
         O.leapfrog(orc, w0, np.arange(n_steps + 1) * dt, save_all=True, fast=True)
         cpu = 1024 * n_steps / (time.perf_counter() - c0)
+    # 16 waves (one per SIMD of 16 CUs): each step costs one orbit's
+    # dependency chain, so neither the FP64 pipe nor HBM bounds it
+    flops0 = 1024 * n_steps * roofline.FLOP_LEAPFROG_STEP
+    bytes0 = 1024 * (n_steps + 1) * roofline.BYTES_SAVED_STATE
     out["n1024_save_all"] = {"n_orbits": 1024, "n_steps": n_steps, "dt": dt, "save_stride": 1,
                              "value": 1024 * n_steps / wall, "unit": "orbit-steps/s",
                              "kernel_ms": ev, "cpu_value": cpu,
-                             "cpu_cores": int(O.num_threads(fast=True))}
+                             "cpu_cores": int(O.num_threads(fast=True)),
+                             "gpu_over_cpu": (1024 * n_steps / wall) / cpu if cpu else None,
+                             "roofline": {"bound": "latency", "pipe": "fp64 (VALU)",
+                                          "achieved": flops0 / (ev / 1e3) / 1e12,
+                                          "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                          "frac": flops0 / (ev / 1e3) / 1e12
+                                          / roofline.PEAK_FP64_TFLOPS,
+                                          "hbm_write_GBps": bytes0 / (ev / 1e3) / 1e9,
+                                          "hbm_frac": bytes0 / (ev / 1e3) / 1e9
+                                          / roofline.PEAK_HBM_GBS}}
     n = args.lf_n or 1048576
     steps = 1000
     w0, _, _ = workloads.composite_leapfrog(pot.circular_velocity, n=n, seed=0)
@@ -239,7 +280,7 @@ def bench_leapfrog(args):
     out["n1M_final"] = {
         "n_orbits": n, "n_steps": steps, "dt": dt, "value": n * steps / wall_f,
         "unit": "orbit-steps/s", "kernel_ms": ev_f,
-        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)", "achieved": flops / (ev_f / 1e3) / 1e12,
+        "roofline": {"bound": "fp64-valu", "pipe": "fp64 (VALU)", "achieved": flops / (ev_f / 1e3) / 1e12,
                      "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s"},
         "cpu_value": cpu, "cpu_cores": int(O.num_threads(fast=True)),
         "cpu_sample": f"first {ns} orbits, {steps} steps, final state only"}
@@ -269,11 +310,16 @@ def bench_mockstream(args):
     prog_w0, prog_mass, b, dt, nsteps = workloads.mockstream_progenitor()
     pot = MilkyWayLikePotential()
     out = {}
+    from gala_amd import roofline
     for integ in ("leapfrog", "dop853"):
+        gens = []
+
         def run():
             df = FardalStreamDF(gala_modified=True, random_state=np.random.RandomState(1))
             gen = MockStreamGenerator(df, pot,
                                       progenitor_potential=PlummerPotential(m=prog_mass, b=b))
+            gen.record_kernel_time = True
+            gens.append(gen)
             return gen.run(prog_w0, prog_mass, dt=dt, n_steps=nsteps, Integrator=integ)
         run()
         torch.cuda.synchronize()
@@ -283,10 +329,25 @@ def bench_mockstream(args):
         wall = time.perf_counter() - t0
         n = stream.xyz.shape[1]
         # particle-steps of the reference's fixed-step loop: release i runs
-        # nsteps - i steps for each of its particles (and its progenitor copy)
+        # nsteps - i steps for each of its particles; each release's
+        # progenitor copy steps alongside (mockstream.pyx:584-590)
         psteps = int(2 * sum(range(nsteps + 1)))
         out[integ] = {"particles": n, "wall_s": wall,
                       "particle_steps_per_s": psteps / wall if integ == "leapfrog" else None}
+        ev = gens[-1].kernel_events
+        if integ == "leapfrog" and ev is not None:
+            kms = ev[0].elapsed_time(ev[1])
+            flops = psteps * roofline.FLOP_STREAM_PARTICLE_STEP + \
+                (psteps // 2) * roofline.FLOP_LEAPFROG_STEP
+            out[integ]["kernel_ms"] = kms
+            out[integ]["kernel_particle_steps_per_s"] = psteps / (kms / 1e3)
+            # ~125 waves for 8,002 particles: one orbit's step chain per wave
+            # bounds the kernel, not the FP64 pipe
+            out[integ]["roofline"] = {
+                "bound": "latency", "pipe": "fp64 (VALU)",
+                "kernel": "k_mockstream (release-and-integrate)",
+                "achieved": flops / (kms / 1e3) / 1e12, "peak": roofline.PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": flops / (kms / 1e3) / 1e12 / roofline.PEAK_FP64_TFLOPS}
         log(f"mock stream ({integ}): {n} particles, {wall * 1e3:.1f} ms end to end")
     if not args.no_cpu:
         from oracle import oracle as O
@@ -331,11 +392,28 @@ def bench_scf(args):
              "value": n * n_steps / wall, "unit": "orbit-steps/s", "wall_s": wall,
              "kernel_ms": ev, "dE_over_E": {"median": float(np.median(d_exact)),
                                             "p99": float(np.percentile(d_exact, 99))},
-             "roofline": {"bound": "mfma", "pipe": "fp64 (VALU)",
+             "roofline": {"bound": "fp64-valu", "pipe": "fp64 (VALU)",
                           "achieved": n * n_steps * roofline.FLOP_SCF_LEAPFROG_STEP / (ev / 1e3) / 1e12,
                           "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}}
     exact["roofline"]["frac"] = exact["roofline"]["achieved"] / roofline.PEAK_FP64_TFLOPS
     log(f"exact SCF leapfrog: {exact['value']:.3e} orbit-steps/s")
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ns, nst = min(n, 4096), 200
+        orc = [O.scf(1e12, 20.0, S, T)]
+        c0 = time.perf_counter()
+        O.leapfrog(orc, np.ascontiguousarray(w0[:, :ns]), np.arange(nst + 1) * dt,
+                   save_all=False, fast=True)
+        cdt = time.perf_counter() - c0
+        exact["cpu_baseline"] = {
+            "value": ns * nst / cdt, "unit": "orbit-steps/s", "cores": int(O.num_threads(fast=True)),
+            "kind": "port", "sample": f"first {ns} orbits of the same workload, {nst} steps, final "
+                                      f"state only ({cdt:.1f} s), oracle/gala_oracle.c built "
+                                      f"-Ofast with OpenMP (gala's SCF series with GSL's special "
+                                      f"functions replaced by recurrences)",
+            "host": host_cpu()}
+        exact["gpu_over_cpu"] = exact["value"] / exact["cpu_baseline"]["value"]
+        log(f"exact SCF leapfrog CPU: {exact['cpu_baseline']['value']:.3e} orbit-steps/s")
 
     sur = SCFSurrogate(pot, seed=0)
     (hist), t_train, _ = _timed(lambda: sur.train(n_points=args.train_points, batch=32768,

@@ -407,6 +407,10 @@ class MockStreamGenerator:
             raise TypeError("If specified, the progenitor_potential must be a potential instance.")
         self.self_gravity = progenitor_potential is not None
         self.progenitor_potential = progenitor_potential
+        # when set, run() records HIP events around the release-and-integrate
+        # kernel (Leapfrog) into kernel_events, for timing on the launch stream
+        self.record_kernel_time = False
+        self.kernel_events = None
 
     def _system(self, pw0, nbody):
         """The bodies integrated with the stream (_get_nbody,
@@ -485,11 +489,18 @@ class MockStreamGenerator:
         reld = dev.to_dev(rel_t)
         tfinal = float(orbit_t[-1])
         handles = _lib.body_array([None if b is None else b.c_handle for b in bodies])
+        if self.record_kernel_time:
+            t_ = dev.torch()
+            self.kernel_events = (t_.cuda.Event(enable_timing=True),
+                                  t_.cuda.Event(enable_timing=True))
+            self.kernel_events[0].record(t_.cuda.current_stream())
         _lib.check(lib.gamd_mockstream_leapfrog(pot.c_handle, handles, K, traj_dev.data_ptr(),
                                                 nfull, float(orbit_t[0]), float(dt),
                                                 w0d.data_ptr(), reld.data_ptr(), n, tfinal,
                                                 out.data_ptr(), dev.stream_ptr()),
                    "gamd_mockstream_leapfrog")
+        if self.record_kernel_time:
+            self.kernel_events[1].record(dev.torch().cuda.current_stream())
         raw_stream = dev.to_host(out)
         traj = dev.to_host(traj_dev)
         # the bodies the reference returns: those of the last release batch,

@@ -31,6 +31,11 @@ FLOP_DOP853_DENSE_EXTRA = 1140
 FLOP_SCF_GRAD = 4222
 FLOP_SCF_LEAPFROG_STEP = 4240
 FLOP_MLP_FORWARD = 2 * (4 * 256 + 3 * 256 * 256 + 256 * 4)
+# mock-stream particle-step: composite gradient 74, shift to the progenitor
+# 3, Plummer gradient 15, leapfrog update 18 (mockstream.pyx:584-599)
+FLOP_STREAM_PARTICLE_STEP = 110
+# bytes a saved leapfrog step writes per orbit (six doubles)
+BYTES_SAVED_STATE = 48
 BYTES_LEAPFROG_SAVE = 48  # one (x, v) state written per saved orbit-step
 
 # MI355X published peaks

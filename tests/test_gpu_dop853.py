@@ -17,15 +17,36 @@ from tests.helpers import mw_oracle, mw_product, state_rel_err, vcirc_oracle
 pytestmark = pytest.mark.gpu
 
 
+def _energy(orc, w):
+    return O.value(orc, np.ascontiguousarray(w[:3])) + 0.5 * np.sum(w[3:] ** 2, axis=0)
+
+
 def test_per_orbit_uneven_parity():
-    from gala_amd.integrate import dop853
+    """16,384 orbits of the 1,048,576-orbit uneven workload over their FULL
+    integration times (up to 10 Gyr): the 1,024 longest by the pilot pass
+    (the orbits that set the launch's length, up to ~23,700 step attempts)
+    and 15,360 drawn at random from the rest.
+
+    Over these times a third of the orbits (those through the 0.07 kpc
+    nucleus) are chaotic: in the oracle itself a one-ulp change of the
+    initial state, or building it with gala's own -Ofast instead of IEEE
+    flags, moves their final states by up to O(1).  Orbits insensitive to
+    both (response <= 1e-10) must match within 1e-8 (the stated DOP853 bar);
+    the chaotic ones cannot match any ulp-different implementation and are
+    held to the oracle's energy-error distribution instead.  Status codes
+    must be identical for all, and the share of orbits whose accepted-step
+    count differs may not exceed what a one-ulp change causes in the oracle."""
     from gala_amd import _device as dev
+    from gala_amd.integrate import dop853, pilot_order
     orc = mw_oracle()
-    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=2048, seed=1)
-    T = T * 0.2  # keep the oracle run short
+    w_all, T_all = workloads.composite_dop853(vcirc_oracle(orc), n=1048576, seed=1)
+    pot = mw_product()
+    order = dev.to_host(pilot_order(pot, w_all, 0.0, T_all, 1.0, rtol=1e-8, atol=1e-10))
+    rest = np.random.default_rng(2).choice(order[1024:], 15360, replace=False)
+    sel = np.sort(np.concatenate([order[:1024], rest]))
+    w0, T = np.ascontiguousarray(w_all[:, sel]), np.ascontiguousarray(T_all[sel])
+    longest = np.isin(sel, order[:1024])
     ref, st_ref, stats_ref = O.dop853_orbits(orc, w0, 0.0, T, 1.0, atol=1e-10, rtol=1e-8)
-    # each orbit's own sensitivity: the oracle again from w0 moved by one ulp
-    # in x, in y and in v_z (the largest of the three responses)
     sens = np.zeros(w0.shape[1])
     for k, direction in ((0, np.inf), (1, -np.inf), (5, np.inf)):
         pert = w0.copy()
@@ -34,24 +55,42 @@ def test_per_orbit_uneven_parity():
         sens = np.maximum(sens, state_rel_err(alt, ref))
         if k == 0:
             stats_alt = stats_k
-    tol = np.maximum(1e-8, 100.0 * sens)
-    res = dop853(mw_product(), w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
+    build = state_rel_err(O.dop853_orbits(orc, w0, 0.0, T, 1.0, atol=1e-10, rtol=1e-8,
+                                          fast=True)[0], ref)
+    calm = (sens <= 1e-10) & (build <= 1e-10)
+    res = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
     w = dev.to_host(res["w_final"])
     st = dev.to_host(res["status"])
     stats = dev.to_host(res["stats"])
     assert np.array_equal(st, st_ref)
     same = stats[:, 1] == stats_ref[:, 1]
     own = (stats_alt[:, 1] == stats_ref[:, 1]).mean()
-    print(f"identical accepted-step counts: GPU {same.mean():.4f}, oracle under a one-ulp "
-          f"change {own:.4f}; {np.sum(tol > 1e-8)} orbit(s) gated by their own sensitivity")
-    assert 1.0 - same.mean() <= 3.0 * (1.0 - own) + 0.005
+    flip = np.any(stats[:, :3] != stats_ref[:, :3], axis=1)
     err = state_rel_err(w, ref)
-    bad = np.nonzero(err > tol)[0]
+    e0 = _energy(orc, w0)
+    de_gpu = np.abs(_energy(orc, w) - e0) / np.abs(e0)
+    de_ref = np.abs(_energy(orc, ref) - e0) / np.abs(e0)
+    ch = ~calm
+    print(f"{len(sel)} orbits, max attempts {stats_ref[:, 0].max()}; identical accepted-step "
+          f"counts GPU {same.mean():.4f}, oracle under a one-ulp change {own:.4f}.  "
+          f"{calm.sum()} insensitive orbits ({calm.mean():.1%}; of the longest 1,024: "
+          f"{calm[longest].sum()}): error median {np.median(err[calm]):.2e}, max "
+          f"{err[calm].max():.2e}, {flip[calm].sum()} with other step counts.  {ch.sum()} "
+          f"chaotic: |dE/E| median GPU {np.median(de_gpu[ch]):.2e} oracle "
+          f"{np.median(de_ref[ch]):.2e}, 99% {np.percentile(de_gpu[ch], 99):.2e} / "
+          f"{np.percentile(de_ref[ch], 99):.2e}, max {de_gpu[ch].max():.2e} / {de_ref[ch].max():.2e}")
+    bad = np.nonzero(calm & (err > 1e-8))[0]
     for i in bad[:10]:
-        print(f"orbit {i}: err {err[i]:.3e} tol {tol[i]:.3e} naccpt gpu {stats[i, 1]} "
-              f"oracle {stats_ref[i, 1]} |r0| {np.linalg.norm(w0[:3, i]):.3f} T {T[i]:.1f}")
+        print(f"orbit {sel[i]}: err {err[i]:.3e} sens {sens[i]:.1e} build {build[i]:.1e} "
+              f"naccpt gpu {stats[i, 1]} oracle {stats_ref[i, 1]} T {T[i]:.1f}")
+    assert 1.0 - same.mean() <= 3.0 * (1.0 - own) + 0.005
+    assert calm.mean() >= 0.55
     assert len(bad) == 0
-    assert np.median(err) < 1e-10
+    assert np.median(err[calm]) < 1e-12
+    assert flip[calm].mean() <= 0.01
+    assert np.median(de_gpu[ch]) <= 2.0 * np.median(de_ref[ch])
+    assert np.percentile(de_gpu[ch], 99) <= 2.0 * np.percentile(de_ref[ch], 99)
+    assert de_gpu[ch].max() <= 3.0 * de_ref[ch].max()
 
 
 def test_dense_output_parity():

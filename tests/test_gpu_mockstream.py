@@ -97,6 +97,7 @@ def test_release_and_integrate_kernel_matches_oracle():
     print(f"max rel err {err.max():.3e}; {np.sum(tol > 1e-10)} particle(s) gated by their own "
           f"sensitivity")
     assert np.all(err <= tol)
+    assert np.mean(tol > 1e-10) <= 0.06
 
 
 def test_full_generator_invariants():
@@ -158,15 +159,24 @@ def test_full_size_stream_against_oracle():
     pw[0] = np.nextafter(pw[0], np.inf)
     _, sf3 = O.mockstream_leapfrog(args[0], args[1], full_t, pw, rel_t, counts, ic, full_t[-1])
     sens = np.maximum(state_rel_err(sf2.T, sf.T), state_rel_err(sf3.T, sf.T))
+    # bar: 1e-9 rather than the orbit Leapfrog's 1e-10.  Particles released
+    # within a few r_J of the 10 pc Plummer progenitor feel its force grow
+    # like 1/d^2 on close passes, which amplifies the few-ulp force
+    # differences of the GPU's Milky-Way evaluator step after step; the
+    # fraction above 1e-10 is printed.
     tol = np.maximum(1e-9, 100 * sens)
     err = state_rel_err(stream.w(), sf.T)
     bad = np.nonzero(err > tol)[0]
+    gated = tol > 1e-9
     print(f"median {np.median(err):.3e}, 99% {np.percentile(err, 99):.3e}, max {err.max():.3e}; "
-          f"{np.sum(tol > 1e-9)} particle(s) gated by their own sensitivity, {len(bad)} over")
+          f"{np.mean(err > 1e-10):.2%} above 1e-10; {gated.sum()} particle(s) ({gated.mean():.2%}) "
+          f"gated by their own sensitivity, {len(bad)} over")
     for i in bad[:8]:
         print(f"  particle {i}: err {err[i]:.3e} tol {tol[i]:.3e} release {ics.release_time[i]}")
     assert np.median(err) < 1e-10
     assert len(bad) == 0
+    assert gated.mean() <= 0.06
+    assert np.mean(err > 1e-10) <= 0.05
     # the progenitor returns to (15, 0, 0) kpc: compare it relative to |x|, |v|
     # (its near-zero y and z components differ at the 1e-12 kpc level)
     assert state_rel_err(prog.w()[:, 0:1], pf[:, None])[0] < 1e-10
@@ -328,6 +338,7 @@ def test_dop853_generator_against_oracle_pipeline():
           f"{np.sum(tol > 1e-8)} particle(s) gated")
     assert np.median(err) < 1e-9
     assert np.all(err <= tol)
+    assert np.mean(tol > 1e-8) <= 0.06
     assert state_rel_err(prog.w()[:, 0:1], ref_pf[:, None])[0] < 1e-8
 
 

@@ -293,6 +293,8 @@ def test_generator_with_perturber_against_oracle_pipeline(integrator):
         floor = 1e-8
     err = state_rel_err(stream.w(), ref_out.T)
     tol = _gate(ref_out.T, alt.T, floor)
-    print(f"{integrator} stream with perturber: median {np.median(err):.3e}, max {err.max():.3e}")
+    print(f"{integrator} stream with perturber: median {np.median(err):.3e}, max {err.max():.3e}; "
+          f"{np.sum(tol > floor)} of {len(tol)} particle(s) gated by their own sensitivity")
     assert np.all(err <= tol)
+    assert np.mean(tol > floor) <= 0.10
     assert np.all(state_rel_err(bodies_f.w(), ref_bf.T) < floor * 100)
