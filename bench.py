@@ -427,6 +427,7 @@ def bench_scf(args):
     res, wall_s, ev_s = _timed(lambda: sur.leapfrog(w0d, dt, n_steps, tol=args.tol), stream)
     fb = dev.to_host(res["fallback_step"])
     d_sur = rel_drift(res["w_final"])
+    on = fb < 0
     n_short = max(1, n_steps // 10)
     res2, wall_p, ev_p = _timed(lambda: sur.leapfrog(w0d, dt, n_short, tol=1e300), stream)
     d_pure = rel_drift(res2["w_final"])
@@ -439,13 +440,24 @@ def bench_scf(args):
                           "p99": float(np.percentile(e_f, 99)), "points": int(qs.shape[1])},
         "fallback_tol": args.tol, "value": n * n_steps / wall_s, "unit": "orbit-steps/s",
         "wall_s": wall_s, "fraction_fallen_back": float(np.mean(fb >= 0)),
+        "median_switch_step": float(np.median(fb[~on])) if np.any(~on) else None,
+        # energy errors against the FP64 SCF (the monitor bounds the
+        # network's own energy Phi_net + |v|^2/2, not this one)
         "dE_over_E": {"median": float(np.median(d_sur)), "p99": float(np.percentile(d_sur, 99))},
+        "dE_over_E_kept_on_network": {
+            "median": float(np.median(d_sur[on])) if np.any(on) else None,
+            "p90": float(np.percentile(d_sur[on], 90)) if np.any(on) else None,
+            "p99": float(np.percentile(d_sur[on], 99)) if np.any(on) else None},
+        "note": "the 4x256 network costs ~190 times the FP64 nmax=10, lmax=6 expansion per "
+                "evaluation (its conservative force needs a forward and a reverse pass); at "
+                "the 32:1 BF16:FP64 peak ratio of the MI355X it cannot be faster than the "
+                "exact kernel, let alone 5x",
         "surrogate_only": {"n_steps": n_short, "value": n * n_short / wall_p,
                            "exact_value_same_steps": n * n_short / wall_e2,
                            "dE_over_E": {"median": float(np.median(d_pure)),
                                          "p99": float(np.percentile(d_pure, 99))},
                            "roofline": {"bound": "mfma", "pipe": "bf16 MFMA",
-                                        "achieved": n * n_short * roofline.FLOP_MLP_FORWARD
+                                        "achieved": n * n_short * roofline.FLOP_MLP_FORCE
                                         / (ev_p / 1e3) / 1e12,
                                         "peak": roofline.PEAK_BF16_TFLOPS, "unit": "TFLOP/s"}},
         "speedup_vs_exact": (n * n_steps / wall_s) / exact["value"],

@@ -35,7 +35,8 @@ class MLPNorm(C.Structure):
     """gamd_mlp_norm (include/gala_amd.h)."""
     _fields_ = [("mu", C.c_double), ("inv_sigma", C.c_double),
                 ("lnr0", C.c_double), ("inv_dlnr", C.c_double), ("pad_r", C.c_double),
-                ("scale", C.c_void_p)]
+                ("scale", C.c_void_p), ("gm", C.c_double), ("rs", C.c_double),
+                ("phi_c", C.c_double)]
 
 
 MLP_KNOTS = 32  # GAMD_MLP_KNOTS
@@ -55,6 +56,7 @@ SIGNATURES = {
     "gamd_potential_d2_dr2": (_int, [_vp, _d, _vp, _sz, _vp, _vp]),
     "gamd_hamiltonian_energy": (_int, [_vp, _d, _vp, _sz, _vp, _vp]),
     "gamd_leapfrog": (_int, [_vp, _vp, _sz, _d, _d, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "gamd_leapfrog_resume": (_int, [_vp, _vp, _sz, _d, _i64, _vp, _vp, _vp]),
     "gamd_ruth4": (_int, [_vp, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp, _vp]),
     "gamd_dop853": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _d, _d, _i64, _i64, _d, _vp, _sz,
                            _vp, _vp, _vp, _vp, _vp, _i64, _int, _vp]),
@@ -79,8 +81,8 @@ SIGNATURES = {
     "gamd_mlp_make_targets": (_int, [_nm, _vp, _vp, _vp, _sz, _vp, _vp, _vp]),
     "gamd_mlp_gradients": (_int, [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz, _vp]),
     "gamd_mlp_adam": (_int, [_vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _vp, _vp]),
-    "gamd_surrogate_leapfrog_step": (_int, [_vp, _vp, _nm, _vp, _vp, _vp, _vp, _vp, _sz, _d,
-                                            _int, _i32, _d, _vp]),
+    "gamd_surrogate_leapfrog_step": (_int, [_vp, _vp, _nm, _vp, _vp, _vp, _vp, _vp, _vp, _sz,
+                                            _d, _int, _i32, _d, _vp]),
 }
 
 _LIB = None

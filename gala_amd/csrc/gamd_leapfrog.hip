@@ -28,7 +28,9 @@ using namespace gamd;
 
 namespace {
 
-template <bool SCF, bool SAVE, int LAY>
+// RESUME: w0 holds positions and HALF-STEP velocities (the state the loop
+// carries between steps), so the initial half kick is skipped
+template <bool SCF, bool SAVE, int LAY, bool RESUME = false>
 __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp, const double *__restrict__ w0,
                                                   int64_t N, double dt, int64_t n_steps,
                                                   const int64_t *__restrict__ nsteps_i,
@@ -51,8 +53,13 @@ __global__ __launch_bounds__(256) void k_leapfrog(const DevPot *__restrict__ Pp,
         wout[5 * s + i] = vz;
     }
     double gx, gy, gz;
-    pot_grad_l<SCF, LAY>(P, x, y, z, gx, gy, gz);
-    double hx = vx - gx * dt / 2.0, hy = vy - gy * dt / 2.0, hz = vz - gz * dt / 2.0;
+    double hx = vx, hy = vy, hz = vz;
+    if (!RESUME) {
+        pot_grad_l<SCF, LAY>(P, x, y, z, gx, gy, gz);
+        hx = vx - gx * dt / 2.0;
+        hy = vy - gy * dt / 2.0;
+        hz = vz - gz * dt / 2.0;
+    }
     int64_t next_save = stride, o = i;
     for (int64_t j = 1; j <= ns; j++) {
         x = x + hx * dt;
@@ -176,6 +183,25 @@ extern "C" int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t
                                 : k_leapfrog<false, false, LAYOUT_GENERIC>);
     hipLaunchKernelGGL(kern, dim3(g), dim3(LF_BLOCK), 0, s, pot->ddev, w0, (int64_t)N, dt, n_steps,
                        n_steps_per_orbit, save_stride, nsave, w_out, w_final);
+    return launch_status();
+}
+
+extern "C" int gamd_leapfrog_resume(const gamd_potential *pot, const double *w_half, size_t N,
+                                    double dt, int64_t n_steps, const int64_t *n_steps_per_orbit,
+                                    double *w_final, void *stream) {
+    if (!pot || !w_half || !w_final || n_steps < 0) return GAMD_ERR_INVALID;
+    if (N == 0) return GAMD_OK;
+    const unsigned g = grid_for(N, LF_BLOCK);
+    const bool scf = pot->dev.has_scf;
+    const bool mw = pot->dev.layout == LAYOUT_MW;
+    const bool scf106 = pot->dev.layout == LAYOUT_SCF106;
+    auto kern = scf106 ? k_leapfrog<true, false, LAYOUT_SCF106, true>
+                : scf  ? k_leapfrog<true, false, LAYOUT_GENERIC, true>
+                : mw   ? k_leapfrog<false, false, LAYOUT_MW, true>
+                       : k_leapfrog<false, false, LAYOUT_GENERIC, true>;
+    hipLaunchKernelGGL(kern, dim3(g), dim3(LF_BLOCK), 0, as_stream(stream), pot->ddev, w_half,
+                       (int64_t)N, dt, n_steps, n_steps_per_orbit, (int64_t)1, (int64_t)0,
+                       (double *)nullptr, w_final);
     return launch_status();
 }
 

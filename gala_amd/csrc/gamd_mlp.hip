@@ -1,33 +1,46 @@
-// gamd_mlp.hip -- the learned SCF-force surrogate: a 4 -> 256 x 4 (SiLU) -> 4
-// MLP in BF16 on the matrix cores (FP32 accumulation, FP32 master weights),
-// its training kernels (forward, backward, Adam) and the surrogate leapfrog
-// with a per-orbit energy monitor and exact-SCF fallback.  No counterpart in
-// gala: the exact force it stands in for is the SCF expansion of
-// potential/scf/src/bfe.cpp:114-189.
+// gamd_mlp.hip -- the learned SCF surrogate: a 4 -> 256 x 4 (SiLU) -> 4 MLP
+// in BF16 on the matrix cores (FP32 accumulation, FP32 master weights), its
+// training kernels (forward, backward, Adam) and the surrogate leapfrog with
+// a per-orbit energy monitor and exact-SCF fallback.  No counterpart in gala:
+// the exact potential it stands in for is the SCF expansion of
+// potential/scf/src/bfe.cpp:20-259.
 //
 // Network, per point q (FP64 in, FP64 out):
 //   features  f = (x/r, y/r, z/r, (ln r - mu)/sigma): direction and log
 //             radius, so small radii are resolved as finely as large ones
 //   hidden    h1 = silu(f W1 + b1), h_l = silu(h_{l-1} W_l + b_l), l = 2..4
 //   output    o = h4 W5 + b5   (4 values)
-//   physics   Phi = s_phi(r) o0,  grad = s_grad(r) (o1, o2, o3)
-// with s_phi, s_grad radial scale profiles (piecewise linear in ln r), so
-// every output is of order one at every radius: with these expansions the
-// force varies by orders of magnitude over 1-200 kpc and departs from the
-// monopole (S_000) force by factors of several, so neither raw nor
-// monopole-relative targets train well.
+//   physics   Phi_net(q) = Phi_b(r) (1 + c o0), with Phi_b = -G M / (r + r_s)
+//             the expansion's Hernquist monopole (smooth, so Phi_net is);
+//             the orbit force is -grad Phi_net, the exact gradient of this
+//             potential (reverse mode through the network), so a surrogate
+//             orbit conserves Phi_net + |v|^2/2 up to the integrator's error
+//             and the energy monitor measures a conservative system.
+//   o1..o3    auxiliary training targets, s_grad(r) o[1:4] ~ grad Phi (the
+//             exact force scaled by a radial profile, piecewise linear in
+//             ln r): they shape the hidden features during training; the
+//             orbit force does not use them.
+//
+// Two evaluations of the same network:
+//   * training (tile_forward, k_mlp_forward_train): BF16 activations and
+//     weights, 128-point tiles;
+//   * orbits and inference (tile_conservative): the hidden products in
+//     three BF16 matrix-core products, hi*hi + hi*lo + lo*hi of the split
+//     v = hi + lo (hi = bf16(v), lo = bf16(v - hi)), which carries ~16
+//     significant bits: the BF16 network's rounding jitter (~1e-3 of o0)
+//     would drift the monitored energy by more than 1e-5 within steps.
 //
 // Matrix-core mapping (gfx950, v_mfma_f32_16x16x32_bf16; operand layout
 // A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8);
 // D[i][j] in lane j + 16 (i/4), element i%4):
-//   * a workgroup (4 waves) owns a tile of 128 points; the activations of a
-//     layer live in LDS (128 x 256 BF16, rows padded to 264 to keep the
-//     16-byte fragment reads conflict-free), double buffered;
-//   * wave w computes output columns [64w, 64w+64) of all 128 rows: 8 x 4
-//     MFMA tiles, 128 FP32 accumulators per lane, 32 independent MFMAs per
-//     K step;
-//   * hidden-layer weights are pre-swizzled into B-fragment order, so each
-//     wave reads 1 KB contiguous per (N tile, K step) from L2.
+//   * training: a workgroup (4 waves) owns a tile of 128 points; a layer's
+//     activations live in LDS (128 x 256 BF16, rows padded to 264 to keep
+//     the 16-byte fragment reads conflict-free); wave w computes output
+//     units [64w, 64w+64) of all 128 rows: 8 x 4 MFMA tiles, 128 FP32
+//     accumulators per lane;
+//   * hidden-layer weights are pre-swizzled into fragment order (and, for
+//     the reverse pass, their transposes), so each wave reads 1 KB
+//     contiguous per (16-unit tile, K step) from L2.
 #include <math.h>
 #include <string.h>
 
@@ -50,8 +63,14 @@ constexpr int OFF_W3 = OFF_B2 + H, OFF_B3 = OFF_W3 + H * H;
 constexpr int OFF_W4 = OFF_B3 + H, OFF_B4 = OFF_W4 + H * H;
 constexpr int OFF_W5 = OFF_B4 + H, OFF_B5 = OFF_W5 + H * 4;
 constexpr int NPARAM = OFF_B5 + 4;  // 199,684
-// BF16 copies: swizzled W2..W4, then row-major W2..W4
-constexpr int NBF = 6 * H * H;
+// BF16 copies of W2..W4, each block 3 H^2 (one H^2 per layer): fragment
+// order (hi), row-major (hi, for the training backward), fragment order of
+// the low parts, and fragment order of the transposes (hi, then lo) for the
+// reverse pass of tile_conservative
+constexpr int BF_SW = 0, BF_ROW = 3 * H * H, BF_SWLO = 6 * H * H, BF_TSW = 9 * H * H,
+              BF_TSWLO = 12 * H * H;
+constexpr int NBF = 15 * H * H;
+constexpr int CP = 32;  // points per workgroup of tile_conservative
 
 // logistic function from the hardware exp2 and reciprocal (v_exp_f32,
 // v_rcp_f32, ~1 ulp each): the activations are rounded to BF16 (8 bits)
@@ -81,19 +100,44 @@ __device__ __forceinline__ void features(const gamd_mlp_norm &nm, double x, doub
     f[3] = (float)((log(r) - nm.mu) * nm.inv_sigma);
 }
 
-// the output scales at radius r: linear interpolation in ln r between the
-// profile knots, constant beyond the end knots (numpy.interp's convention)
-__device__ __forceinline__ void out_scales(const gamd_mlp_norm &nm, double r, double &sphi,
-                                           double &sgrad) {
+// the auxiliary heads' scale at radius r: linear interpolation in ln r
+// between the profile knots, constant beyond the end knots (numpy.interp's
+// convention)
+__device__ __forceinline__ double out_scale_grad(const gamd_mlp_norm &nm, double r) {
     double u = (log(r) - nm.lnr0) * nm.inv_dlnr;
     if (!(u > 0.0)) u = 0.0;  // also maps NaN (r = 0 with a zero knot spacing) to the first knot
     if (u > (double)(GAMD_MLP_KNOTS - 1)) u = (double)(GAMD_MLP_KNOTS - 1);
     int k = (int)u;
     k = k > GAMD_MLP_KNOTS - 2 ? GAMD_MLP_KNOTS - 2 : k;
     const double f = u - (double)k;
-    const double *sp = nm.scale, *sg = nm.scale + GAMD_MLP_KNOTS;
-    sphi = sp[k] + f * (sp[k + 1] - sp[k]);
-    sgrad = sg[k] + f * (sg[k + 1] - sg[k]);
+    const double *sg = nm.scale;
+    return sg[k] + f * (sg[k + 1] - sg[k]);
+}
+
+// Phi_net and its gradient at q = (x, y, z), |q| = r, from the network's o0
+// and its derivatives g[c] = d o0 / d f_c (f the features): with
+// f = (q/r, (ln r - mu)/sigma), d(q_i/r)/dq_k = (delta_ik - q_i q_k / r^2) / r
+// and d f_3/dq_k = q_k / (r^2 sigma).
+__device__ __forceinline__ void net_physics(const gamd_mlp_norm &nm, double x, double y, double z,
+                                            double r, float o0, const float *g, double &phi,
+                                            double &gx, double &gy, double &gz) {
+    const double rb = r + nm.rs;
+    const double pb = -nm.gm / rb;            // Phi_b
+    const double dpb = nm.gm / (rb * rb);     // d Phi_b / dr
+    const double ratio = 1.0 + nm.phi_c * (double)o0;
+    phi = pb * ratio;
+    const double ir = 1.0 / r;
+    const double g0 = (double)g[0], g1 = (double)g[1], g2 = (double)g[2], g3 = (double)g[3];
+    const double gq = (g0 * x + g1 * y + g2 * z) * ir * ir;
+    const double gr = g3 * nm.inv_sigma * ir * ir;
+    // d o0 / dq
+    const double ox = (g0 - gq * x) * ir + gr * x;
+    const double oy = (g1 - gq * y) * ir + gr * y;
+    const double oz = (g2 - gq * z) * ir + gr * z;
+    const double a = dpb * ratio * ir, b = pb * nm.phi_c;
+    gx = a * x + b * ox;
+    gy = a * y + b * oy;
+    gz = a * z + b * oz;
 }
 
 // ---------------------------------------------------------------------------
@@ -212,48 +256,222 @@ __device__ __forceinline__ void tile_forward(const float *__restrict__ theta, co
 constexpr int LDS_BYTES = TILE * LDSR * 2;
 
 // ---------------------------------------------------------------------------
-// inference: q (3,M) FP64 -> Phi (M), grad (3,M)
-__global__ __launch_bounds__(256, 2) void k_mlp_forward(const float *__restrict__ theta,
-                                                     const __bf16 *__restrict__ wsw,
+// The precise evaluation of one CP-point tile: o0 and d o0 / d f (reverse
+// mode), the hidden products in three BF16 matrix-core products of the
+// hi/lo split (hi*hi + hi*lo + lo*hi), SiLU and its derivative in FP32, the
+// first and last layers in FP32 on the VALU.
+//   feat   LDS [CP][4] features;
+//   bhi,   LDS [CP][LDSR] BF16 each: a layer's input (activations forward,
+//   blo    error signals backward), split;
+//   red    LDS [4][CP][5] per-wave partial sums;
+//   o0,    LDS [CP] and [CP][4]: results.
+//   dfo
+// Wave w owns units [64w, 64w + 64) of every layer (forward and backward),
+// so the SiLU derivatives it keeps in registers from the forward pass are
+// exactly those its reverse-pass products need.  All 256 threads call it.
+
+__device__ __forceinline__ void split_bf(float v, __bf16 &hi, __bf16 &lo) {
+    hi = tobf(v);
+    lo = tobf(v - frombf(hi));
+}
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// acc[mi][ni] (+)= sum over the 256 inputs of the layer's weights (fragment
+// order: hi at Whi, lo at Wlo) times the split input in bhi/blo
+__device__ __forceinline__ void split_layer(const __bf16 *__restrict__ Whi,
+                                            const __bf16 *__restrict__ Wlo, const __bf16 *bhi,
+                                            const __bf16 *blo, f32x4 (&acc)[2][4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int ks = 0; ks < 8; ks++) {
+        bf16x8 wh[4], wl[4];
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++) {
+            const size_t o = (((size_t)(4 * w + ni) * 8 + ks) * 64 + lane) * 8;
+            wh[ni] = *reinterpret_cast<const bf16x8 *>(Whi + o);
+            wl[ni] = *reinterpret_cast<const bf16x8 *>(Wlo + o);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++) {
+            const int off = (16 * mi + (lane & 15)) * LDSR + 32 * ks + 8 * (lane >> 4);
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(bhi + off);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(blo + off);
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++) {
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ni], ah, acc[mi][ni], 0, 0, 0);
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[ni], al, acc[mi][ni], 0, 0, 0);
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[ni], ah, acc[mi][ni], 0, 0, 0);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_conservative(const float *__restrict__ theta,
+                                                  const __bf16 *__restrict__ wbf,
+                                                  const float (*feat)[4], __bf16 *bhi, __bf16 *blo,
+                                                  float (*red)[CP][5], float *o0, float (*dfo)[4]) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // layer 1 (FP32): thread t owns unit t
+    {
+        const int j = t;
+        const float w0 = theta[OFF_W1 + 0 * H + j], w1 = theta[OFF_W1 + 1 * H + j];
+        const float w2 = theta[OFF_W1 + 2 * H + j], w3 = theta[OFF_W1 + 3 * H + j];
+        const float bj = theta[OFF_B1 + j];
+#pragma unroll 4
+        for (int p = 0; p < CP; p++) {
+            const float z = bj + feat[p][0] * w0 + feat[p][1] * w1 + feat[p][2] * w2 + feat[p][3] * w3;
+            split_bf(silu(z), bhi[p * LDSR + j], blo[p * LDSR + j]);
+        }
+    }
+    __syncthreads();
+    // SiLU derivatives of layers 2 and 3 at this wave's units, in the
+    // accumulator layout: unit 64w + 16ni + 4(lane/16) + e, point 16mi + lane%16
+    f32x4 ds[2][2][4];
+    float po[2] = {0.f, 0.f};  // partial o0 of points lane%16 and 16 + lane%16
+    f32x4 acc[2][4];
+#pragma unroll 1
+    for (int l = 0; l < 3; l++) {
+        split_layer(wbf + BF_SW + (size_t)l * H * H, wbf + BF_SWLO + (size_t)l * H * H, bhi, blo,
+                    acc);
+        __syncthreads();  // every wave has read the layer's input
+        const float *bias = theta + (l == 0 ? OFF_B2 : l == 1 ? OFF_B3 : OFF_B4);
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++) {
+                const int unit = 64 * w + 16 * ni + 4 * (lane >> 4);
+                const int row = 16 * mi + (lane & 15);
+                const f32x4 bc = *reinterpret_cast<const f32x4 *>(bias + unit);
+                __bf16 hh[4], hl[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float z = acc[mi][ni][e] + bc[e];
+                    const float sg = sigmoid_fast(z);
+                    const float h = z * sg;
+                    const float d = sg * (1.0f + z * (1.0f - sg));
+                    if (l < 2) {
+                        ds[l][mi][ni][e] = d;
+                        split_bf(h, hh[e], hl[e]);
+                    } else {
+                        // layer 4: the output dot and the output's error signal
+                        const float w5 = theta[OFF_W5 + (unit + e) * 4];
+                        po[mi] += w5 * h;
+                        split_bf(w5 * d, hh[e], hl[e]);
+                    }
+                }
+                *reinterpret_cast<bf16x4 *>(bhi + row * LDSR + unit) = bf16x4{hh[0], hh[1], hh[2], hh[3]};
+                *reinterpret_cast<bf16x4 *>(blo + row * LDSR + unit) = bf16x4{hl[0], hl[1], hl[2], hl[3]};
+            }
+        __syncthreads();
+    }
+    // o0: the four lanes of a point in this wave, then the four waves
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++) {
+        float v = po[mi];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (lane < 16) red[w][16 * mi + lane][4] = v;
+    }
+    // reverse pass: delta_{l-1} = (W_l^T delta_l) silu'(z_{l-1}), l = 4, 3, 2
+    float pg[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 1
+    for (int l = 2; l >= 0; l--) {
+        split_layer(wbf + BF_TSW + (size_t)l * H * H, wbf + BF_TSWLO + (size_t)l * H * H, bhi, blo,
+                    acc);
+        __syncthreads();
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++) {
+                const int unit = 64 * w + 16 * ni + 4 * (lane >> 4);
+                const int row = 16 * mi + (lane & 15);
+                __bf16 hh[4], hl[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    float d;
+                    if (l > 0) {
+                        d = acc[mi][ni][e] * ds[l - 1][mi][ni][e];
+                        split_bf(d, hh[e], hl[e]);
+                    } else {
+                        // layer 1's derivative, recomputed from its inputs
+                        const int u = unit + e;
+                        const float z = theta[OFF_B1 + u] + feat[row][0] * theta[OFF_W1 + u] +
+                                        feat[row][1] * theta[OFF_W1 + H + u] +
+                                        feat[row][2] * theta[OFF_W1 + 2 * H + u] +
+                                        feat[row][3] * theta[OFF_W1 + 3 * H + u];
+                        d = acc[mi][ni][e] * dsilu(z);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) pg[mi][c] += theta[OFF_W1 + c * H + u] * d;
+                    }
+                }
+                if (l > 0) {
+                    *reinterpret_cast<bf16x4 *>(bhi + row * LDSR + unit) = bf16x4{hh[0], hh[1], hh[2], hh[3]};
+                    *reinterpret_cast<bf16x4 *>(blo + row * LDSR + unit) = bf16x4{hl[0], hl[1], hl[2], hl[3]};
+                }
+            }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float v = pg[mi][c];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (lane < 16) red[w][16 * mi + lane][c] = v;
+        }
+    __syncthreads();
+    if (t < CP) {
+        o0[t] = theta[OFF_B5] + red[0][t][4] + red[1][t][4] + red[2][t][4] + red[3][t][4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) dfo[t][c] = red[0][t][c] + red[1][t][c] + red[2][t][c] + red[3][t][c];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// inference: q (3,M) FP64 -> Phi_net (M), grad Phi_net (3,M), one CP-point
+// tile per workgroup (tile_conservative)
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ theta,
+                                                     const __bf16 *__restrict__ wbf,
                                                      gamd_mlp_norm nm,
                                                      const double *__restrict__ q, int64_t M,
                                                      double *__restrict__ phi,
                                                      double *__restrict__ grad) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *buf = reinterpret_cast<__bf16 *>(smem);
-    __shared__ float feat[TILE][4];
-    __shared__ float out[TILE][4];
+    __shared__ __align__(16) __bf16 bhi[CP * LDSR];
+    __shared__ __align__(16) __bf16 blo[CP * LDSR];
+    __shared__ float feat[CP][4];
+    __shared__ float red[4][CP][5];
+    __shared__ float o0[CP];
+    __shared__ float dfo[CP][4];
     const int t = threadIdx.x;
-    const int64_t row0 = (int64_t)blockIdx.x * TILE;
-    double x = 0, y = 0, z = 0, r = 1;
-    if (t < TILE) {
-        const int64_t i = row0 + t;
+    const int64_t i = (int64_t)blockIdx.x * CP + t;
+    double x = nm.pad_r, y = 0, z = 0, r = 1;  // padding rows: a harmless point
+    if (t < CP) {
         if (i < M) {
             x = q[i];
             y = q[M + i];
             z = q[2 * M + i];
-        } else {
-            x = nm.pad_r;  // a harmless point for the padding rows
         }
         float f[4];
         features(nm, x, y, z, f, r);
-        feat[t][0] = f[0];
-        feat[t][1] = f[1];
-        feat[t][2] = f[2];
-        feat[t][3] = f[3];
+#pragma unroll
+        for (int c = 0; c < 4; c++) feat[t][c] = f[c];
     }
     __syncthreads();
-    tile_forward<false>(theta, wsw, feat, buf, out, row0, M, nullptr, nullptr);
-    if (t < TILE) {
-        const int64_t i = row0 + t;
-        if (i < M) {
-            double sphi, sgrad;
-            out_scales(nm, r, sphi, sgrad);
-            phi[i] = sphi * (double)out[t][0];
-            grad[i] = sgrad * (double)out[t][1];
-            grad[M + i] = sgrad * (double)out[t][2];
-            grad[2 * M + i] = sgrad * (double)out[t][3];
-        }
+    tile_conservative(theta, wbf, feat, bhi, blo, red, o0, dfo);
+    if (t < CP && i < M) {
+        double ph, gx, gy, gz;
+        net_physics(nm, x, y, z, r, o0[t], dfo[t], ph, gx, gy, gz);
+        phi[i] = ph;
+        grad[i] = gx;
+        grad[M + i] = gy;
+        grad[2 * M + i] = gz;
     }
 }
 
@@ -580,7 +798,16 @@ __global__ __launch_bounds__(256) void k_outer4(const float *__restrict__ feat,
     }
 }
 
-// BF16 copies of W2..W4: swizzled B-fragment order and row-major
+// BF16 copies of W2..W4 (layout at BF_SW .. BF_TSWLO): fragment order of
+// the high parts, row-major high parts, fragment order of the low parts,
+// and fragment order of the transposes' high and low parts
+__device__ __forceinline__ size_t frag_index(int k, int n) {
+    // fragment order of V[k][n] (k input, n output): ((jt * 8 + ks) * 64 +
+    // lane) * 8 + e with n = 16 jt + lane % 16, k = 32 ks + 8 (lane / 16) + e
+    const int jt = n >> 4, ks = k >> 5, lane = (n & 15) + 16 * ((k & 31) >> 3), e = k & 7;
+    return (((size_t)jt * 8 + ks) * 64 + lane) * 8 + e;
+}
+
 __global__ void k_sync_bf16(const float *__restrict__ theta, __bf16 *__restrict__ wbf) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (idx >= 3 * H * H) return;
@@ -588,11 +815,14 @@ __global__ void k_sync_bf16(const float *__restrict__ theta, __bf16 *__restrict_
     const int rem = (int)(idx % (H * H));
     const int k = rem / H, n = rem % H;  // W[k][n], row-major [in][out]
     const float v = theta[(l == 0 ? OFF_W2 : l == 1 ? OFF_W3 : OFF_W4) + rem];
-    // fragment order: ((jt * 8 + ks) * 64 + lane) * 8 + e with
-    // n = 16 jt + lane % 16, k = 32 ks + 8 (lane / 16) + e
-    const int jt = n >> 4, ks = k >> 5, lane = (n & 15) + 16 * ((k & 31) >> 3), e = k & 7;
-    wbf[(size_t)l * H * H + (((size_t)jt * 8 + ks) * 64 + lane) * 8 + e] = tobf(v);
-    wbf[(size_t)(3 + l) * H * H + rem] = tobf(v);
+    __bf16 hi, lo;
+    split_bf(v, hi, lo);
+    const size_t lb = (size_t)l * H * H;
+    wbf[BF_SW + lb + frag_index(k, n)] = hi;
+    wbf[BF_ROW + lb + rem] = hi;
+    wbf[BF_SWLO + lb + frag_index(k, n)] = lo;
+    wbf[BF_TSW + lb + frag_index(n, k)] = hi;    // W^T[n][k] = W[k][n]
+    wbf[BF_TSWLO + lb + frag_index(n, k)] = lo;
 }
 
 // Adam (FP32 masters)
@@ -610,7 +840,8 @@ __global__ void k_adam(float *__restrict__ theta, const float *__restrict__ g,
 }
 
 // training data: positions (3,M) FP64 with exact Phi and grad -> features
-// (M,4) and targets (M,4) in the scaled output units
+// (M,4) and targets (M,4): o0 the potential's departure from the monopole
+// in units of c, o1..o3 the force over its radial scale
 __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
                                const double *__restrict__ phi, const double *__restrict__ grad,
                                int64_t M, float *__restrict__ feat, float *__restrict__ target) {
@@ -622,9 +853,9 @@ __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
     features(nm, x, y, z, f, r);
 #pragma unroll
     for (int c = 0; c < 4; c++) feat[i * 4 + c] = f[c];
-    double sphi, sgrad;
-    out_scales(nm, r, sphi, sgrad);
-    target[i * 4 + 0] = (float)(phi[i] / sphi);
+    const double sgrad = out_scale_grad(nm, r);
+    // o0's target: Phi = Phi_b (1 + c o0) with Phi_b = -G M / (r + r_s)
+    target[i * 4 + 0] = (float)((phi[i] / (-nm.gm / (r + nm.rs)) - 1.0) / nm.phi_c);
     target[i * 4 + 1] = (float)(grad[i] / sgrad);
     target[i * 4 + 2] = (float)(grad[M + i] / sgrad);
     target[i * 4 + 3] = (float)(grad[2 * M + i] / sgrad);
@@ -632,26 +863,29 @@ __global__ void k_make_targets(gamd_mlp_norm nm, const double *__restrict__ q,
 
 // ---------------------------------------------------------------------------
 // surrogate leapfrog, one step per call (this kernel for the orbits on the
-// network, then k_surrogate_fallback for those on the exact force).  Orbit i: w (6,N) holds x and the
-// half-step velocity between launches (the synchronised velocity after the
-// last step); E0[i] the initial energy; fb[i] the step at which the orbit fell
-// back to exact SCF (-1 while on the surrogate).  mode 0: initial half kick;
-// mode 1: drift + kick.  The monitored energy is the surrogate's own
-// Phi + |v|^2/2 against E0; once |E - E0| > tol |E0| the orbit switches to
-// the exact SCF force for the rest of the run (and for the remainder of this
-// step).
-__global__ __launch_bounds__(256, 2) void k_surrogate_step(
-    const float *__restrict__ theta, const __bf16 *__restrict__ wsw, gamd_mlp_norm nm,
+// network, then k_surrogate_fallback for those on the exact force).  Orbit i:
+// w (6,N) holds x and the half-step velocity between launches (vsync the
+// synchronised velocity after the last step); E0[i] the initial energy;
+// fb[i] the step at which the orbit fell back to exact SCF (-1 while on the
+// surrogate).  mode 0: initial half kick; mode 1: drift + kick.  The force
+// is -grad Phi_net, the gradient of the network's own potential, so the
+// monitored energy Phi_net + |v|^2/2 is that of a conservative system; once
+// |E - E0| > tol |E0| the orbit switches to the exact SCF force for the rest
+// of the run (and for the remainder of this step).
+__global__ __launch_bounds__(256) void k_surrogate_step(
+    const float *__restrict__ theta, const __bf16 *__restrict__ wbf, gamd_mlp_norm nm,
     double *__restrict__ wq, double *__restrict__ vsync, double *__restrict__ E0,
-    int32_t *__restrict__ fb, int64_t N, double dt, int mode, int32_t step, double tol) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *buf = reinterpret_cast<__bf16 *>(smem);
-    __shared__ float feat[TILE][4];
-    __shared__ float out[TILE][4];
+    int32_t *__restrict__ fb, double *__restrict__ wsw, int64_t N, double dt, int mode,
+    int32_t step, double tol) {
+    __shared__ __align__(16) __bf16 bhi[CP * LDSR];
+    __shared__ __align__(16) __bf16 blo[CP * LDSR];
+    __shared__ float feat[CP][4];
+    __shared__ float red[4][CP][5];
+    __shared__ float o0[CP];
+    __shared__ float dfo[CP][4];
     const int t = threadIdx.x;
-    const int64_t row0 = (int64_t)blockIdx.x * TILE;
-    const int64_t i = row0 + t;
-    const bool own = t < TILE && i < N;
+    const int64_t i = (int64_t)blockIdx.x * CP + t;
+    const bool own = t < CP && i < N;
     double x = nm.pad_r, y = 0, z = 0, hx = 0, hy = 0, hz = 0, r = 1;
     int32_t fallback = -1;
     if (own) {
@@ -668,46 +902,36 @@ __global__ __launch_bounds__(256, 2) void k_surrogate_step(
             z = z + hz * dt;
         }
     }
-    if (t < TILE) {
+    if (t < CP) {
         float f[4];
         features(nm, x, y, z, f, r);
-        feat[t][0] = f[0];
-        feat[t][1] = f[1];
-        feat[t][2] = f[2];
-        feat[t][3] = f[3];
+#pragma unroll
+        for (int c = 0; c < 4; c++) feat[t][c] = f[c];
     }
-    __syncthreads();
     // the tile runs the network unless every orbit in it has fallen back
+    // (uniform over the workgroup, so every thread reaches the same barriers)
     const int any_net = __syncthreads_or(own && fallback < 0);
-    if (any_net) tile_forward<false>(theta, wsw, feat, buf, out, row0, N, nullptr, nullptr);
-    if (!own) return;
-    double gx, gy, gz, phi;
-    if (fallback < 0) {
-        double sphi, sgrad;
-        out_scales(nm, r, sphi, sgrad);
-        phi = sphi * (double)out[t][0];
-        gx = sgrad * (double)out[t][1];
-        gy = sgrad * (double)out[t][2];
-        gz = sgrad * (double)out[t][3];
-    }
-    const double vx0 = (mode == 0) ? hx : 0.0, vy0 = (mode == 0) ? hy : 0.0,
-                 vz0 = (mode == 0) ? hz : 0.0;
-    if (fallback < 0) {
-        // energy monitor on the synchronised velocity this force gives
-        const double vx = (mode == 0) ? vx0 : hx - gx * dt / 2.0;
-        const double vy = (mode == 0) ? vy0 : hy - gy * dt / 2.0;
-        const double vz = (mode == 0) ? vz0 : hz - gz * dt / 2.0;
-        const double E = phi + 0.5 * (vx * vx + vy * vy + vz * vz);
-        if (mode == 0) {
-            E0[i] = E;
-        } else if (fabs(E - E0[i]) > tol * fabs(E0[i])) {
-            fallback = step;
-            fb[i] = step;
+    if (any_net) tile_conservative(theta, wbf, feat, bhi, blo, red, o0, dfo);
+    if (!own || fallback >= 0) return;  // exact-force orbits: k_surrogate_fallback
+    double phi, gx, gy, gz;
+    net_physics(nm, x, y, z, r, o0[t], dfo[t], phi, gx, gy, gz);
+    // energy monitor on the synchronised velocity this force gives
+    const double vx = (mode == 0) ? hx : hx - gx * dt / 2.0;
+    const double vy = (mode == 0) ? hy : hy - gy * dt / 2.0;
+    const double vz = (mode == 0) ? hz : hz - gz * dt / 2.0;
+    const double E = phi + 0.5 * (vx * vx + vy * vy + vz * vz);
+    if (mode == 0) {
+        E0[i] = E;
+    } else if (fabs(E - E0[i]) > tol * fabs(E0[i])) {
+        // from this step on the exact force: k_surrogate_fallback redoes it
+        // from the state before the step, which is recorded when asked
+        fb[i] = step;
+        if (wsw) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) wsw[c * N + i] = wq[c * N + i];
         }
+        return;
     }
-    // orbits on the exact force (already, or from this step on) are stepped
-    // by k_surrogate_fallback, launched right after on the same stream
-    if (fallback >= 0) return;
     if (mode == 0) {
         wq[3 * N + i] = hx - gx * dt / 2.0;
         wq[4 * N + i] = hy - gy * dt / 2.0;
@@ -716,9 +940,9 @@ __global__ __launch_bounds__(256, 2) void k_surrogate_step(
         wq[i] = x;
         wq[N + i] = y;
         wq[2 * N + i] = z;
-        vsync[i] = hx - gx * dt / 2.0;
-        vsync[N + i] = hy - gy * dt / 2.0;
-        vsync[2 * N + i] = hz - gz * dt / 2.0;
+        vsync[i] = vx;
+        vsync[N + i] = vy;
+        vsync[2 * N + i] = vz;
         wq[3 * N + i] = hx - gx * dt;
         wq[4 * N + i] = hy - gy * dt;
         wq[5 * N + i] = hz - gz * dt;
@@ -727,9 +951,12 @@ __global__ __launch_bounds__(256, 2) void k_surrogate_step(
 
 // The same leapfrog step for the orbits on the exact SCF force (fb >= 0,
 // including those k_surrogate_step has just switched): the drift is redone
-// from the stored state with the same arithmetic, then the force, kick and
-// writes as k_surrogate_step would make them.  A kernel of its own keeps the
-// SCF evaluator's registers out of the network kernel.
+// from the stored state, then the force, kick and writes with the
+// arithmetic and the SCF evaluator of gamd_leapfrog's loop (k_leapfrog for
+// the same potential layout), so from its switch step on an orbit follows
+// the exact leapfrog bit for bit.  A kernel of its own keeps the SCF
+// evaluator's registers out of the network kernel.
+template <int LAY>
 __global__ __launch_bounds__(256) void k_surrogate_fallback(const DevPot *__restrict__ Pp,
                                                             double *__restrict__ wq,
                                                             double *__restrict__ vsync,
@@ -744,8 +971,8 @@ __global__ __launch_bounds__(256) void k_surrogate_fallback(const DevPot *__rest
         y = y + hy * dt;
         z = z + hz * dt;
     }
-    double ph, gx = 0.0, gy = 0.0, gz = 0.0;
-    scf_eval(Pp->c[0], x, y, z, true, ph, gx, gy, gz);
+    double gx, gy, gz;
+    pot_grad_l<true, LAY>(*Pp, x, y, z, gx, gy, gz);
     if (mode == 0) {
         wq[3 * N + i] = hx - gx * dt / 2.0;
         wq[4 * N + i] = hy - gy * dt / 2.0;
@@ -766,7 +993,8 @@ __global__ __launch_bounds__(256) void k_surrogate_fallback(const DevPot *__rest
 int check_norm(const gamd_mlp_norm *nm) {
     if (!nm || !nm->scale) return GAMD_ERR_INVALID;
     if (!(nm->inv_dlnr > 0.0) || !isfinite(nm->inv_dlnr) || !isfinite(nm->lnr0) ||
-        !(nm->pad_r > 0.0))
+        !(nm->pad_r > 0.0) || !(nm->gm > 0.0) || !(nm->rs > 0.0) || !(nm->phi_c > 0.0) ||
+        !isfinite(nm->gm) || !isfinite(nm->rs) || !isfinite(nm->phi_c))
         return GAMD_ERR_INVALID;
     return GAMD_OK;
 }
@@ -775,14 +1003,13 @@ int check_params(const float *theta, const uint16_t *wbf) {
     return (theta && wbf) ? GAMD_OK : GAMD_ERR_INVALID;
 }
 
-// the tile kernels use 66 KB of dynamic LDS (one padded 128 x 256 BF16
-// activation buffer), above the default 64 KB; raise their limit once per
+// the training forward uses 66 KB of dynamic LDS (one padded 128 x 256 BF16
+// activation buffer), above the default 64 KB; raise its limit once per
 // process (a function-local static: initialised once even when several host
 // threads make their first call together)
 int ensure_lds() {
     static const int rc = [] {
-        const void *ks[3] = {(const void *)k_mlp_forward, (const void *)k_mlp_forward_train,
-                             (const void *)k_surrogate_step};
+        const void *ks[1] = {(const void *)k_mlp_forward_train};
         for (const void *k : ks)
             if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES) !=
                 hipSuccess)
@@ -817,8 +1044,7 @@ int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_nor
     if (check_params(theta, wbf) || check_norm(norm) || (M && (!q || !phi || !grad)))
         return GAMD_ERR_INVALID;
     if (M == 0) return GAMD_OK;
-    if (ensure_lds()) return GAMD_ERR_HIP;
-    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((M + TILE - 1) / TILE)), dim3(256), LDS_BYTES,
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((M + CP - 1) / CP)), dim3(256), 0,
                        as_stream(stream), theta, reinterpret_cast<const __bf16 *>(wbf), *norm, q,
                        (int64_t)M, phi, grad);
     return launch_status();
@@ -910,8 +1136,8 @@ int gamd_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t s
 
 int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
                                  const gamd_potential *scf, double *w, double *vsync, double *E0,
-                                 int32_t *fallback_step, size_t N, double dt, int mode,
-                                 int32_t step, double tol, void *stream) {
+                                 int32_t *fallback_step, double *w_switch, size_t N, double dt,
+                                 int mode, int32_t step, double tol, void *stream) {
     if (check_params(theta, wbf) || check_norm(norm) || !scf || !w || !vsync || !E0 ||
         !fallback_step)
         return GAMD_ERR_INVALID;
@@ -919,15 +1145,16 @@ int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const 
         return GAMD_ERR_INVALID;
     if (mode != 0 && mode != 1) return GAMD_ERR_INVALID;
     if (N == 0) return GAMD_OK;
-    if (ensure_lds()) return GAMD_ERR_HIP;
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_surrogate_step, dim3((unsigned)((N + TILE - 1) / TILE)), dim3(256),
-                       LDS_BYTES, s, theta, reinterpret_cast<const __bf16 *>(wbf), *norm, w, vsync,
-                       E0, fallback_step, (int64_t)N, dt, mode, step, tol);
+    hipLaunchKernelGGL(k_surrogate_step, dim3((unsigned)((N + CP - 1) / CP)), dim3(256), 0, s,
+                       theta, reinterpret_cast<const __bf16 *>(wbf), *norm, w, vsync, E0,
+                       fallback_step, w_switch, (int64_t)N, dt, mode, step, tol);
     int rc = launch_status();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_surrogate_fallback, dim3(grid_for(N, 256)), dim3(256), 0, s, scf->ddev, w,
-                       vsync, fallback_step, (int64_t)N, dt, mode);
+    auto fk = scf->dev.layout == LAYOUT_SCF106 ? k_surrogate_fallback<LAYOUT_SCF106>
+                                                : k_surrogate_fallback<LAYOUT_GENERIC>;
+    hipLaunchKernelGGL(fk, dim3(grid_for(N, 256)), dim3(256), 0, s, scf->ddev, w, vsync,
+                       fallback_step, (int64_t)N, dt, mode);
     return launch_status();
 }
 

@@ -31,6 +31,11 @@ FLOP_DOP853_DENSE_EXTRA = 1140
 FLOP_SCF_GRAD = 4222
 FLOP_SCF_LEAPFROG_STEP = 4240
 FLOP_MLP_FORWARD = 2 * (4 * 256 + 3 * 256 * 256 + 256 * 4)
+# the orbit force of the surrogate: Phi_net's forward (o0 only) and the
+# reverse pass for d o0 / d features (three hidden layers and the first
+# layer's 4 x 256 reduction); the matrix cores execute the hidden products
+# three times (hi*hi + hi*lo + lo*hi of the BF16 split), not counted here
+FLOP_MLP_FORCE = 2 * (4 * 256 + 3 * 256 * 256 + 256) + 2 * (3 * 256 * 256 + 4 * 256)
 # mock-stream particle-step: composite gradient 74, shift to the progenitor
 # 3, Plummer gradient 15, leapfrog update 18 (mockstream.pyx:584-599)
 FLOP_STREAM_PARTICLE_STEP = 110

@@ -1,17 +1,20 @@
-"""Learned SCF-force surrogate (no gala counterpart): a 4 -> 256 x 4 (SiLU)
--> 4 MLP trained on forces of the exact SCF expansion, on BF16 matrix cores
-with FP32 accumulation and FP32 master weights, used in a leapfrog whose
-per-orbit energy monitor falls back to the exact SCF force.
+"""Learned SCF surrogate (no gala counterpart): a 4 -> 256 x 4 (SiLU) -> 4
+MLP trained on the exact SCF expansion, on BF16 matrix cores with FP32
+accumulation and FP32 master weights, used in a leapfrog whose per-orbit
+energy monitor falls back to the exact SCF force.
 
 Everything runs in the HIP kernels of libgala_amd.so (gamd_mlp_*,
 gamd_surrogate_leapfrog_step); this module owns the device buffers and the
 training loop.
 
-Outputs are scaled by radial profiles of the exact potential: Phi =
-s_phi(r) o0, grad = s_grad(r) o[1:4], with s_phi and s_grad the RMS of the
-exact Phi and of one exact gradient component over shells in ln r (32 knots
-between rmin and rmax, hat-function weights), so every target is of order
-one at every radius.
+The network's potential is Phi_net = Phi_b(r) (1 + c o0) with Phi_b =
+-G M / (r + r_s) the expansion's Hernquist monopole and c the RMS of the
+exact Phi / Phi_b - 1 over the training radii, so o0 is of order one; the
+orbit force is -grad Phi_net (reverse mode through the network), so
+surrogate orbits conserve Phi_net + |v|^2/2 up to the leapfrog's error.
+Outputs o1..o3 are auxiliary targets, grad Phi / s_grad(r) with s_grad the
+RMS of one exact gradient component over shells in ln r (32 knots between
+rmin and rmax, hat-function weights).
 """
 import ctypes as C
 import math
@@ -68,9 +71,12 @@ class SCFSurrogate:
         self.pot = scf_potential
         self.rmin, self.rmax = rmin, rmax
         lo, hi = math.log(rmin), math.log(rmax)
-        self.scale = self._scale_profile(lo, hi)
+        self.gm = float(scf_potential.G * scf_potential.parameters["m"])
+        self.rs = float(scf_potential.parameters["r_s"])
+        self.scale, self.phi_c = self._scale_profile(lo, hi)
         self.norm = MLPNorm(0.5 * (lo + hi), math.sqrt(12.0) / (hi - lo), lo,
-                            (_lib.MLP_KNOTS - 1) / (hi - lo), 10.0, self.scale.data_ptr())
+                            (_lib.MLP_KNOTS - 1) / (hi - lo), 10.0, self.scale.data_ptr(),
+                            self.gm, self.rs, self.phi_c)
         self.theta = dev.to_dev(init_params(seed), dtype=t.float32)
         self.wbf = t.empty(lib.gamd_mlp_bf16_count(), dtype=t.bfloat16, device="cuda")
         self.m = t.zeros_like(self.theta)
@@ -96,22 +102,24 @@ class SCFSurrogate:
         return qd, phi, grad
 
     def _scale_profile(self, lo, hi, n=1 << 16, seed=20240):
-        """(2, K) device table: RMS of the exact Phi and of one exact gradient
-        component around each of the K knots in ln r (hat-function weights)."""
+        """(K,) device table of the RMS of one exact gradient component around
+        each of the K knots in ln r (hat-function weights), and c, the RMS of
+        Phi / Phi_b - 1 over the sample."""
         K = _lib.MLP_KNOTS
         q = self.sample_points(n, seed)
         _, phi, grad = self._exact(q)
         phi, grad = dev.to_host(phi), dev.to_host(grad)
-        u = (np.log(np.linalg.norm(q, axis=0)) - lo) * ((K - 1) / (hi - lo))
+        r = np.linalg.norm(q, axis=0)
+        u = (np.log(r) - lo) * ((K - 1) / (hi - lo))
         wts = np.maximum(0.0, 1.0 - np.abs(u[:, None] - np.arange(K)[None, :]))
         den = np.maximum(wts.sum(axis=0), 1e-300)
-        sp = np.sqrt((wts * (phi ** 2)[:, None]).sum(axis=0) / den)
         sg = np.sqrt((wts * (np.sum(grad ** 2, axis=0) / 3.0)[:, None]).sum(axis=0) / den)
-        table = np.stack([sp, sg])
-        if not np.all(np.isfinite(table)) or np.any(table <= 0.0):
-            raise RuntimeError("degenerate output scale profile (zero potential or force "
-                               "over a shell)")
-        return dev.to_dev(table)
+        if not np.all(np.isfinite(sg)) or np.any(sg <= 0.0):
+            raise RuntimeError("degenerate force scale profile (zero force over a shell)")
+        c = float(np.sqrt(np.mean((phi / (-self.gm / (r + self.rs)) - 1.0) ** 2)))
+        if not (np.isfinite(c) and c > 0.0):
+            c = 1.0
+        return dev.to_dev(sg), c
 
     def _sync(self):
         _lib.check(self.lib.gamd_mlp_sync_bf16(self.theta.data_ptr(), self.wbf.data_ptr(),
@@ -199,7 +207,8 @@ class SCFSurrogate:
 
     # ------------------------------------------------------------------
     def forward(self, q):
-        """Phi (M,) and grad (3, M) from the network at q (3, M)."""
+        """Phi_net (M,) and grad Phi_net (3, M) at q (3, M): the network's
+        potential and its exact gradient (the negative orbit force)."""
         qd = dev.to_dev(q)
         M = qd.shape[1]
         phi = dev.empty((M,))
@@ -210,26 +219,33 @@ class SCFSurrogate:
                    "gamd_mlp_forward")
         return phi, grad
 
-    def leapfrog(self, w0, dt, n_steps, tol=1e-5, progress=None, check_every=100):
-        """Leapfrog of orbits w0 (6, N) with surrogate forces and per-orbit
-        exact-SCF fallback once |dE/E| of the surrogate energy exceeds tol
-        (tol < 0: every orbit on the exact force from the start).  Every
-        check_every steps, once every orbit has fallen back, the remaining
-        steps run in the persistent exact leapfrog kernel (gamd_leapfrog)
-        from the synchronised state.
-        Returns dict(w_final (6,N) device, fallback_step (N,) device int32)."""
+    def leapfrog(self, w0, dt, n_steps, tol=1e-5, progress=None, check_every=100,
+                 record_switch=False):
+        """Leapfrog of orbits w0 (6, N) with the surrogate force -grad Phi_net
+        and per-orbit exact-SCF fallback once |dE/E| of the surrogate energy
+        Phi_net + |v|^2/2 exceeds tol (tol < 0: every orbit on the exact
+        force from the start).  From its switch step on an orbit follows the
+        exact leapfrog bit for bit; every check_every steps, once every orbit
+        has switched, the remaining steps run in the exact leapfrog kernel
+        resumed from the carried state (gamd_leapfrog_resume).
+        Returns dict(w_final (6,N) device, fallback_step (N,) device int32
+        [, w_switch (6,N): each switched orbit's state before its switch
+        step, position and half-step velocity])."""
         t = dev.torch()
         w = dev.to_dev(w0).clone()
         N = w.shape[1]
         vsync = dev.empty((3, N))
         E0 = t.zeros(N, dtype=t.float64, device="cuda")
         fb = t.full((N,), 0 if tol < 0 else -1, dtype=t.int32, device="cuda")
+        wsw = t.full((6, N), float("nan"), dtype=t.float64, device="cuda") if record_switch \
+            else None
         s = dev.stream_ptr()
         fn = self.lib.gamd_surrogate_leapfrog_step
         args = (self.theta.data_ptr(), self.wbf.data_ptr(), C.byref(self.norm),
                 self.pot.c_handle, w.data_ptr(), vsync.data_ptr(), E0.data_ptr(), fb.data_ptr(),
-                N, float(dt))
+                dev.ptr(wsw), N, float(dt))
         _lib.check(fn(*args, 0, 0, float(tol), s), "surrogate init")
+        out = None
         for j in range(1, n_steps + 1):
             rc = fn(*args, 1, j, float(tol), s)
             if rc:
@@ -238,9 +254,15 @@ class SCFSurrogate:
                 progress(j)
             if j < n_steps and check_every and j % check_every == 0 and \
                     not bool((fb < 0).any().item()):
-                from .integrate import leapfrog as exact_leapfrog
-                ws = t.cat([w[:3], vsync], dim=0).contiguous()
-                _, wf = exact_leapfrog(self.pot, ws, dt, n_steps - j, save=False)
-                return {"w_final": wf, "fallback_step": fb}
-        out = t.cat([w[:3], vsync], dim=0) if n_steps > 0 else w
-        return {"w_final": out, "fallback_step": fb}
+                out = dev.empty((6, N))
+                _lib.check(self.lib.gamd_leapfrog_resume(self.pot.c_handle, w.data_ptr(), N,
+                                                         float(dt), n_steps - j, None,
+                                                         out.data_ptr(), s),
+                           "gamd_leapfrog_resume")
+                break
+        if out is None:
+            out = t.cat([w[:3], vsync], dim=0) if n_steps > 0 else w
+        res = {"w_final": out, "fallback_step": fb}
+        if record_switch:
+            res["w_switch"] = wsw
+        return res

@@ -112,6 +112,19 @@ int gamd_leapfrog(const gamd_potential *pot, const double *w0, size_t N, double 
                   int64_t n_steps, const int64_t *n_steps_per_orbit, int64_t save_stride,
                   double *w_out, double *w_final, void *stream);
 
+/* Resume a leapfrog integration from the state its loop carries between
+ * steps (leapfrog.pyx:106-116): positions and HALF-STEP velocities w_half
+ * (6,N), so no initial half kick.  n_steps steps, or n_steps_per_orbit[i]
+ * (device, nullable); w_final (6,N) receives the positions and the
+ * synchronised velocities after the last step (an orbit with 0 steps keeps
+ * its input).  Arithmetic as gamd_leapfrog's loop, so resuming from the
+ * state after step j of a run continues that run bit for bit.  No gala
+ * counterpart (gala cannot resume); the learned surrogate hands orbits on
+ * the exact force to it. */
+int gamd_leapfrog_resume(const gamd_potential *pot, const double *w_half, size_t N, double dt,
+                         int64_t n_steps, const int64_t *n_steps_per_orbit, double *w_final,
+                         void *stream);
+
 /* ---- Ruth4 ------------------------------------------------------------- */
 /* Replaces ruth4_integrate_hamiltonian (ruth4.pyx:24-106): fixed dt, per step
  * four kick-drift substeps with Ruth's coefficients (:67-80).  Arguments and
@@ -292,27 +305,35 @@ int gamd_mockstream_dop853(const gamd_potential *ext, const gamd_potential *cons
  * gamd_mlp_param_count() values, layout W1[4][256] b1[256] W2[256][256] b2
  * W3 b3 W4 b4 W5[256][4] b5[4] (W[in][out]); BF16 copies wbf of
  * gamd_mlp_bf16_count() values made by gamd_mlp_sync_bf16.
- * Features of a point q: (x/r, y/r, z/r, (ln r - mu)/sigma); outputs o give
- * Phi = s_phi(r) o0 and grad = s_grad(r) o[1:4], where s_phi and s_grad are
- * radial scale profiles (typically the RMS of the exact Phi and of one
- * gradient component over shells), piecewise linear in ln r between
- * GAMD_MLP_KNOTS knots at ln r = lnr0 + k / inv_dlnr and constant beyond the
- * end knots. */
+ * Features of a point q: (x/r, y/r, z/r, (ln r - mu)/sigma).  The network's
+ * potential is Phi_net(q) = Phi_b(r) (1 + phi_c o0) with the Hernquist
+ * monopole Phi_b = -gm / (r + rs) of the expansion, and the orbit force is
+ * -grad Phi_net (exact reverse-mode derivative of that potential, so
+ * surrogate orbits are a conservative system).  Outputs o1..o3 are auxiliary
+ * training targets, grad Phi / s_grad(r), with s_grad a radial scale profile
+ * (typically the RMS of one exact gradient component over shells),
+ * piecewise linear in ln r between GAMD_MLP_KNOTS knots at ln r = lnr0 +
+ * k / inv_dlnr and constant beyond the end knots. */
 #define GAMD_MLP_KNOTS 32
 typedef struct {
     double mu, inv_sigma;    /* ln r feature: (ln r - mu) * inv_sigma */
-    double lnr0, inv_dlnr;   /* knot positions of the output scale profiles */
+    double lnr0, inv_dlnr;   /* knot positions of the s_grad profile */
     double pad_r;            /* any positive radius, used for padding rows */
-    const double *scale;     /* DEVICE [2][GAMD_MLP_KNOTS]: s_phi knots, then s_grad knots */
+    const double *scale;     /* DEVICE [GAMD_MLP_KNOTS]: s_grad knots */
+    double gm, rs;           /* the monopole Phi_b = -gm / (r + rs), gm = G M */
+    double phi_c;            /* scale of o0: Phi_net = Phi_b (1 + phi_c o0) */
 } gamd_mlp_norm;
 
 int64_t gamd_mlp_param_count(void);
 int64_t gamd_mlp_bf16_count(void);
 /* bytes of workspace gamd_mlp_gradients needs for a batch of M points */
 int64_t gamd_mlp_workspace_bytes(int64_t M);
-/* BF16 copies (matrix-core fragment order and row-major) of the masters */
+/* BF16 copies of the hidden weights (high and low parts in matrix-core
+ * fragment order, their transposes, and row-major high parts) */
 int gamd_mlp_sync_bf16(const float *theta, uint16_t *wbf, void *stream);
-/* inference: q (3,M) FP64 -> phi (M), grad (3,M) FP64 */
+/* inference: q (3,M) FP64 -> Phi_net (M) and grad Phi_net (3,M) FP64, the
+ * hidden products carried to ~16 significant bits (three BF16 products of
+ * a high/low split) */
 int gamd_mlp_forward(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
                      const double *q, size_t M, double *phi, double *grad, void *stream);
 /* training data: exact phi (M) and grad (3,M) at q (3,M) -> features
@@ -334,14 +355,17 @@ int gamd_mlp_adam(float *theta, const float *grad, float *m, float *v, int64_t s
  * holds positions and half-step velocities between calls; vsync (3,N) the
  * synchronised velocity after the step; E0 (N) the initial energy (written
  * by mode 0, the initial half kick); fallback_step (N) is -1 while an orbit
- * uses the surrogate and the step at which it switched to the exact SCF force
- * of `scf` (a single unshifted SCF component) when its surrogate energy
- * Phi + |v|^2/2 drifted by more than tol |E0|.  mode 0 is the initial half
+ * uses the surrogate force -grad Phi_net and the step at which it switched
+ * to the exact SCF force of `scf` (a single unshifted SCF component) when its
+ * surrogate energy Phi_net + |v|^2/2 drifted by more than tol |E0|; from
+ * that step on the orbit follows gamd_leapfrog's arithmetic and SCF
+ * evaluator exactly, and w_switch (6,N, nullable) receives its state before
+ * the step (position and half-step velocity, gamd_leapfrog_resume's input).  mode 0 is the initial half
  * kick (and sets E0), mode 1 one drift + kick. */
 int gamd_surrogate_leapfrog_step(const float *theta, const uint16_t *wbf, const gamd_mlp_norm *norm,
                                  const gamd_potential *scf, double *w, double *vsync, double *E0,
-                                 int32_t *fallback_step, size_t N, double dt, int mode,
-                                 int32_t step, double tol, void *stream);
+                                 int32_t *fallback_step, double *w_switch, size_t N, double dt,
+                                 int mode, int32_t step, double tol, void *stream);
 
 /* Library version string (host). */
 const char *gamd_version(void);
