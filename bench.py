@@ -292,7 +292,58 @@ def bench_leapfrog(args):
                            "hbm_write_GBps": saved / (ev_s / 1e3) / 1e9}
     log(f"leapfrog composite: 1024x{n_steps} save_all {out['n1024_save_all']['value']:.3e}/s, "
         f"{n}x{steps} {out['n1M_final']['value']:.3e}/s (CPU {cpu}/s)")
+    out["to_host"] = bench_leapfrog_to_host(args, pot, log)
     return out
+
+
+def bench_leapfrog_to_host(args, pot, log):
+    """Saved trajectories delivered to host memory (what integrate_orbit
+    returns): direct writes (the whole trajectory in HBM, then one copy to
+    pinned host memory) against gamd_leapfrog_stream (chunks copied on a
+    second stream while the next chunk computes; chunk size and graph capture
+    autotuned), for the 1,024-orbit save_all case and 1M orbits saved every
+    16th step.  Wall clock to a complete host buffer."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import autotune, workloads
+    from gala_amd.integrate import leapfrog, leapfrog_to_host
+    res = {}
+    for name, n, steps, stride in (("n1024_save_all", 1024, 10000, 1),
+                                   ("n1M_stride16", args.lf_n or 1048576, 1000, 16)):
+        w0, dt, _ = workloads.composite_leapfrog(pot.circular_velocity, n=n, seed=0)
+        w0d = dev.to_dev(w0)
+        nsave = steps // stride + 1
+        host = torch.empty((6, nsave, n), dtype=torch.float64, pin_memory=True)
+
+        def direct():
+            wout, _ = leapfrog(pot, w0d, dt, steps, save_stride=stride)
+            host.copy_(wout)
+            torch.cuda.synchronize()
+
+        tuned = autotune.tune_leapfrog_stream(pot, w0d, dt, steps, stride, log=log)
+
+        def streamed():
+            leapfrog_to_host(pot, w0d, dt, steps, stride, out=host)
+
+        direct()
+        t_d, t_s = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            direct()
+            t_d.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            streamed()
+            t_s.append(time.perf_counter() - t0)
+        gb = 48.0 * nsave * n / 1e9
+        res[name] = {"n_orbits": n, "n_steps": steps, "save_stride": stride, "GB": gb,
+                     "direct_ms": 1e3 * min(t_d), "streamed_ms": 1e3 * min(t_s),
+                     "streamed_GBps": gb / min(t_s), "chunk_rows": tuned["chunk_rows"],
+                     "graph": tuned["graph"], "autotune_ms": tuned["trials_ms"]}
+        log(f"leapfrog to host ({name}, {gb:.2f} GB): direct {1e3 * min(t_d):.1f} ms, "
+            f"streamed {1e3 * min(t_s):.1f} ms ({tuned['chunk_rows']} rows/chunk, graph "
+            f"{int(tuned['graph'])})")
+        del host
+    return res
 
 
 def bench_mockstream(args):

@@ -57,6 +57,9 @@ SIGNATURES = {
     "gamd_hamiltonian_energy": (_int, [_vp, _d, _vp, _sz, _vp, _vp]),
     "gamd_leapfrog": (_int, [_vp, _vp, _sz, _d, _d, _i64, _vp, _i64, _vp, _vp, _vp]),
     "gamd_leapfrog_resume": (_int, [_vp, _vp, _sz, _d, _i64, _vp, _vp, _vp]),
+    "gamd_leapfrog_stream_workspace_bytes": (_i64, [_sz, _i64]),
+    "gamd_leapfrog_stream": (_int, [_vp, _vp, _sz, _d, _d, _i64, _i64, _i64, _int, _vp, _vp, _vp,
+                                    _sz, _vp]),
     "gamd_ruth4": (_int, [_vp, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp, _vp]),
     "gamd_dop853": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _d, _d, _i64, _i64, _d, _vp, _sz,
                            _vp, _vp, _vp, _vp, _vp, _i64, _int, _vp]),

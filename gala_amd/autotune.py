@@ -1,4 +1,5 @@
-"""Launch-shape autotuner for the persistent DOP853 kernel.
+"""Launch-shape autotuner: the persistent DOP853 kernel's workgroup size and
+dedicated rows, and the streamed leapfrog output's chunk size.
 
 gamd_dop853 gives identical results for every workgroup size and every
 number of dedicated-row orbits (n_tail), so these are pure speed knobs.  The
@@ -122,5 +123,80 @@ def tune_dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, order=No
                 log(f"autotune dop853: block {best_b}, n_tail {nt}: {ms:.1f} ms")
     best_t = min(tails, key=lambda nt: trials[f"block={best_b},n_tail={nt}"])
     entry = {"block": int(best_b), "n_tail": int(best_t), "trials_ms": trials}
+    _store(k, entry)
+    return dict(entry, cached=False)
+
+
+# ---------------------------------------------------------------------------
+# streamed leapfrog output (gamd_leapfrog_stream): rows per chunk and graph
+# capture are speed knobs (identical results); the trials run the caller's
+# own integration into its own host buffer
+
+CHUNK_MB = (8, 32, 128, 512)   # candidate chunk sizes (MB of saved rows)
+
+
+def chunk_rows_for(N, nsave, mb):
+    """rows per chunk holding about `mb` MB of saved rows (6 doubles per
+    orbit and row), at least 1, at most nsave"""
+    rows = int(mb * 1e6 // (48 * max(int(N), 1)))
+    return max(1, min(int(nsave), rows))
+
+
+def default_chunk_rows(N, nsave):
+    """untuned choice: ~16 chunks of at least 8 MB (enough chunks for the
+    copies to overlap the kernels, few enough launches to stay cheap)"""
+    return max(chunk_rows_for(N, nsave, 8), -(-int(nsave) // 16))
+
+
+def key_stream(potential, N, nsave, stride):
+    t = dev.torch()
+    name = t.cuda.get_device_name(0) if t.cuda.is_available() else "nogpu"
+    comps = ",".join(str(c["type"]) for c in potential._components())
+    return f"{name}|leapfrog_stream|pot[{comps}]|N={int(N)}|nsave={int(nsave)}|stride={int(stride)}"
+
+
+def lookup_stream(potential, N, nsave, stride):
+    """(chunk_rows, graph) remembered for this problem, or None."""
+    k = key_stream(potential, N, nsave, stride)
+    e = _MEM.get(k) or _load().get(k)
+    return None if e is None else (int(e["chunk_rows"]), bool(e["graph"]))
+
+
+def tune_leapfrog_stream(potential, w0, dt, n_steps, save_stride=1, mbs=CHUNK_MB, repeats=2,
+                         force=False, log=None):
+    """Pick (chunk_rows, graph) for gamd_leapfrog_stream by timing whole runs
+    of this integration (wall clock: the call returns when the host buffer is
+    complete).  Returns dict(chunk_rows, graph, trials_ms)."""
+    import time
+    from .integrate import leapfrog_to_host
+    N = w0.shape[1]
+    nsave = int(n_steps) // int(save_stride) + 1
+    k = key_stream(potential, N, nsave, save_stride)
+    if not force:
+        e = _MEM.get(k) or _load().get(k)
+        if e is not None:
+            return dict(e, cached=True)
+    t = dev.torch()
+    buf = t.empty((6, nsave, N), dtype=t.float64, pin_memory=True)
+    cands = sorted({chunk_rows_for(N, nsave, mb) for mb in mbs})
+    trials = {}
+    leapfrog_to_host(potential, w0, dt, min(int(n_steps), int(save_stride)), save_stride,
+                     chunk_rows=1)  # first-launch costs out of the trials
+    for rows in cands:
+        for graph in (False, True):
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                leapfrog_to_host(potential, w0, dt, n_steps, save_stride, chunk_rows=rows,
+                                 graph=graph, out=buf)
+                ts.append((time.perf_counter() - t0) * 1e3)
+            trials[f"chunk_rows={rows},graph={int(graph)}"] = ms = sum(ts) / len(ts)
+            if log:
+                log(f"autotune leapfrog stream: {rows} rows per chunk, graph {int(graph)}: "
+                    f"{ms:.1f} ms")
+    best = min(trials, key=trials.get)
+    rows = int(best.split(",")[0].split("=")[1])
+    graph = best.endswith("graph=1")
+    entry = {"chunk_rows": rows, "graph": graph, "trials_ms": trials}
     _store(k, entry)
     return dict(entry, cached=False)

@@ -14,7 +14,7 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
-           "dop853_batched",
+           "dop853_batched", "leapfrog_to_host",
            "pilot_order", "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
 
 
@@ -123,6 +123,45 @@ def leapfrog(potential, w0, dt, n_steps, save_stride=1, save=True, n_steps_per_o
                                  int(n_steps), dev.ptr(nsp), int(save_stride), dev.ptr(wout),
                                  wfin.data_ptr(), dev.stream_ptr()), "gamd_leapfrog")
     return wout, wfin
+
+
+def leapfrog_to_host(potential, w0, dt, n_steps, save_stride=1, t0=0.0, chunk_rows=None,
+                     graph=None, out=None):
+    """Leapfrog with the saved trajectory streamed to pinned host memory while
+    the integration runs (gamd_leapfrog_stream): chunks of chunk_rows saved
+    rows are computed and copied on two streams, so the copies overlap the
+    kernels.  chunk_rows / graph: None = the autotuner's remembered choice
+    for this problem, else a default of ~16 chunks.  out: a pinned host
+    float64 tensor (6, nsave, N) to reuse.  Returns (w_out numpy view of the
+    pinned buffer (6, nsave, N), w_final (6, N) device), equal bit for bit to
+    leapfrog(..., save_stride)."""
+    from . import autotune
+    t = dev.require_gpu()
+    lib = _lib.load()
+    w0 = dev.phase_space(w0)
+    N = w0.shape[1]
+    n_steps, save_stride = int(n_steps), int(save_stride)
+    if n_steps < 0 or save_stride < 1:
+        raise ValueError("n_steps must be >= 0 and save_stride >= 1")
+    nsave = n_steps // save_stride + 1
+    if chunk_rows is None or graph is None:
+        tuned = autotune.lookup_stream(potential, N, nsave, save_stride)
+        if chunk_rows is None:
+            chunk_rows = tuned[0] if tuned else autotune.default_chunk_rows(N, nsave)
+        if graph is None:
+            graph = tuned[1] if tuned else False
+    chunk_rows = max(1, min(int(chunk_rows), nsave))
+    if out is None or tuple(out.shape) != (6, nsave, N):
+        out = t.empty((6, nsave, N), dtype=t.float64, pin_memory=True)
+    wfin = dev.empty((6, N))
+    nbytes = int(lib.gamd_leapfrog_stream_workspace_bytes(N, chunk_rows))
+    ws = t.empty(max(nbytes, 8), dtype=t.uint8, device="cuda")
+    _lib.check(lib.gamd_leapfrog_stream(potential.c_handle, w0.data_ptr(), N, float(t0),
+                                        float(dt), n_steps, save_stride, chunk_rows,
+                                        int(bool(graph)), out.data_ptr(), wfin.data_ptr(),
+                                        ws.data_ptr(), nbytes, dev.stream_ptr()),
+               "gamd_leapfrog_stream")
+    return out.numpy(), wfin
 
 
 def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
@@ -356,9 +395,12 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
         if len(t) < 2:
             raise ValueError("need at least two times")
         dt = t[1] - t[0]
-        wout, wfin = leapfrog(pot, arr, dt, len(t) - 1, save_stride=1, save=bool(save_all),
-                              t0=t[0])
-        w = dev.to_host(wout) if save_all else dev.to_host(wfin)
+        if save_all:
+            # the trajectory streams to pinned host memory while it is computed
+            w, _ = leapfrog_to_host(pot, arr, dt, len(t) - 1, save_stride=1, t0=t[0])
+        else:
+            _, wfin = leapfrog(pot, arr, dt, len(t) - 1, save=False, t0=t[0])
+            w = dev.to_host(wfin)
     elif Integrator is Ruth4Integrator:
         if len(t) < 2:
             raise ValueError("need at least two times")

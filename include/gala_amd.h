@@ -125,6 +125,25 @@ int gamd_leapfrog_resume(const gamd_potential *pot, const double *w_half, size_t
                          int64_t n_steps, const int64_t *n_steps_per_orbit, double *w_final,
                          void *stream);
 
+/* Leapfrog whose saved trajectory is streamed to HOST memory while the
+ * integration runs (gala's leapfrog_integrate_hamiltonian returns it on the
+ * host, leapfrog.pyx:91-121).  The nsave = n_steps / save_stride + 1 saved
+ * rows go to host_out (HOST, ideally pinned, (6, nsave, N)) in chunks of
+ * chunk_rows rows: each chunk is one kernel writing a slot of a two-slot
+ * device ring, then copied to the host on a second stream while the next
+ * chunk computes.  Results equal gamd_leapfrog's w_out and w_final (device,
+ * (6,N)) bit for bit.  use_graph != 0 captures the chunk kernels and copies
+ * as one HIP graph and launches that (no memset nodes).  workspace: device,
+ * gamd_leapfrog_stream_workspace_bytes(N, chunk_rows) bytes.  The call
+ * returns when host_out is complete (it synchronises `stream`).  No gala
+ * counterpart for the streaming itself; the chunk size is a speed knob
+ * (gala_amd.autotune picks it). */
+int64_t gamd_leapfrog_stream_workspace_bytes(size_t N, int64_t chunk_rows);
+int gamd_leapfrog_stream(const gamd_potential *pot, const double *w0, size_t N, double t0,
+                         double dt, int64_t n_steps, int64_t save_stride, int64_t chunk_rows,
+                         int use_graph, double *host_out, double *w_final, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
 /* ---- Ruth4 ------------------------------------------------------------- */
 /* Replaces ruth4_integrate_hamiltonian (ruth4.pyx:24-106): fixed dt, per step
  * four kick-drift substeps with Ruth's coefficients (:67-80).  Arguments and

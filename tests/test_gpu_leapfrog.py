@@ -149,3 +149,50 @@ def test_ruth4_integrate_orbit_save_all():
     orbit = Hamiltonian(mw_product()).integrate_orbit(w0, t=t, Integrator="ruth4")
     assert orbit.xyz.shape == (3, 201, 16)
     assert np.max(state_rel_err(orbit.w(), ref)) < 1e-11
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_streamed_output_matches_device_output(graph):
+    """gamd_leapfrog_stream (saved rows computed in chunks and copied to
+    pinned host memory while the next chunk runs, optionally as one captured
+    HIP graph) gives gamd_leapfrog's saved trajectory and final state bit for
+    bit: chunk sizes from one row to all rows, a step count that is not a
+    multiple of the stride, zero steps, the Milky-Way evaluator and the
+    generic component loop.  The graph path is run three times in a row with
+    other GPU work and synchronisations between, every run checked."""
+    from gala_amd import _device as dev
+    from gala_amd.integrate import leapfrog, leapfrog_to_host
+    from gala_amd.potential import HernquistPotential, PlummerPotential
+    orc = mw_oracle()
+    w0, dt, _ = workloads.composite_leapfrog(vcirc_oracle(orc), n=700, seed=4)
+    generic = HernquistPotential(m=1e11, c=2.0) + PlummerPotential(m=1e10, b=1.0,
+                                                                   origin=[1.0, 0.5, -0.2])
+    print(f"streamed leapfrog, graph={graph}", flush=True)
+    for pot in (mw_product(), generic):
+        for n_steps, stride in ((1000, 3), (0, 1), (5, 1)):
+            wout, wfin = leapfrog(pot, w0, dt, n_steps, save_stride=stride)
+            ref, ref_fin = dev.to_host(wout), dev.to_host(wfin)
+            nsave = ref.shape[1]
+            for rows in sorted({1, 7, 50, nsave}):
+                reps = 3 if (graph and rows == 7) else 1
+                for _ in range(reps):
+                    got, gfin = leapfrog_to_host(pot, w0, dt, n_steps, stride, chunk_rows=rows,
+                                                 graph=graph)
+                    assert np.array_equal(got, ref), (n_steps, stride, rows)
+                    assert np.array_equal(dev.to_host(gfin), ref_fin), (n_steps, stride, rows)
+                    leapfrog(pot, w0, dt, 10, save=False)  # other work between the runs
+                    dev.torch().cuda.synchronize()
+
+
+def test_integrate_orbit_leapfrog_streams_to_host():
+    """Hamiltonian.integrate_orbit's Leapfrog (gala's default integrator)
+    returns the streamed trajectory: identical to the device path's."""
+    from gala_amd import _device as dev
+    from gala_amd.integrate import leapfrog
+    from gala_amd.potential import Hamiltonian
+    orc = mw_oracle()
+    w0, dt, _ = workloads.composite_leapfrog(vcirc_oracle(orc), n=300, seed=6)
+    t = np.arange(801) * dt
+    orbit = Hamiltonian(mw_product()).integrate_orbit(w0, t=t, Integrator="leapfrog")
+    wout, _ = leapfrog(mw_product(), w0, dt, 800)
+    assert np.array_equal(orbit.w(), dev.to_host(wout))
