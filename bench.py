@@ -191,6 +191,48 @@ def bench_dop853(args):
     return out, (w0, T)
 
 
+def bench_dop853_batched(args, w0_all):
+    """gala's default DOP853 path (Hamiltonian.integrate_orbit passes
+    nbatch=100): the first 262,144 orbits of the uneven workload's initial
+    conditions over one common 1 Gyr (a batch shares its time grid), final
+    states only, nbatch = 100 (gamd_dop853_batched) against nbatch = 1 (the
+    per-orbit persistent kernel) on the same orbits.  Accepted steps count
+    per orbit (a batch's steps times its orbits)."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import roofline
+    from gala_amd.integrate import dop853, dop853_batched
+    from gala_amd.potential import MilkyWayLikePotential
+    pot = MilkyWayLikePotential()
+    n = min(args.n or 262144, w0_all.shape[1], 262144)
+    w0d = dev.to_dev(np.ascontiguousarray(w0_all[:, :n]))
+    T = 1000.0
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, fn in (("nbatch100", lambda: dop853_batched(pot, w0d, 0.0, T, 1.0, nbatch=100,
+                                                           rtol=1e-8, atol=1e-10)),
+                     ("nbatch1", lambda: dop853(pot, w0d, 0.0, T, 1.0, rtol=1e-8, atol=1e-10))):
+        fn()
+        r, wall, ev = _timed(fn, stream)
+        stats = dev.to_host(r["stats"])
+        acc = int(stats[:, 1].sum())
+        rej = int(stats[:, 2].sum())
+        res[name] = {"n_orbits": n, "T": T, "wall_ms": wall * 1e3, "kernel_ms": ev,
+                     "accepted_orbit_steps": acc,
+                     "value": acc / wall, "unit": "orbit-steps/s",
+                     "status_ok": float(np.mean(dev.to_host(r["status"]) == 1))}
+        if name == "nbatch100":
+            flops = roofline.dop853_flops(acc, rej)
+            res[name]["roofline"] = {"bound": "fp64-valu", "pipe": "fp64 (VALU)",
+                                     "achieved": flops / (ev / 1e3) / 1e12,
+                                     "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": flops / (ev / 1e3) / 1e12
+                                     / roofline.PEAK_FP64_TFLOPS}
+        log(f"dop853 {name}: {n} orbits x {T:.0f} Myr in {wall * 1e3:.1f} ms, "
+            f"{acc / wall:.3e} accepted orbit-steps/s")
+    return res
+
+
 def _energy(pot, w):
     """exact H = Phi + |v|^2/2 of device states w (6, N), on device."""
     from gala_amd import _device as dev
@@ -557,6 +599,7 @@ def main():
     out, (w0, T) = bench_dop853(args)
     if not args.no_leapfrog:
         out["leapfrog_composite"] = bench_leapfrog(args)
+    out["dop853_default_batching"] = bench_dop853_batched(args, w0)
     if not args.no_stream:
         out["mockstream"] = bench_mockstream(args)
     if not args.no_scf:
