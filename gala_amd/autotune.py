@@ -1,5 +1,6 @@
 """Launch-shape autotuner: the persistent DOP853 kernel's workgroup size and
-dedicated rows, and the streamed leapfrog output's chunk size.
+dedicated rows, the streamed leapfrog output's chunk size and the streamed
+DOP853 dense output's window size (with or without graph capture).
 
 gamd_dop853 gives identical results for every workgroup size and every
 number of dedicated-row orbits (n_tail), so these are pure speed knobs.  The
@@ -198,5 +199,62 @@ def tune_leapfrog_stream(potential, w0, dt, n_steps, save_stride=1, mbs=CHUNK_MB
     rows = int(best.split(",")[0].split("=")[1])
     graph = best.endswith("graph=1")
     entry = {"chunk_rows": rows, "graph": graph, "trials_ms": trials}
+    _store(k, entry)
+    return dict(entry, cached=False)
+
+
+# ---------------------------------------------------------------------------
+# streamed DOP853 dense output (gamd_dop853_dense_stream): samples per window
+# and graph capture are speed knobs (identical results)
+
+
+def key_dense(potential, N, n_out):
+    t = dev.torch()
+    name = t.cuda.get_device_name(0) if t.cuda.is_available() else "nogpu"
+    comps = ",".join(str(c["type"]) for c in potential._components())
+    return f"{name}|dop853_dense|pot[{comps}]|N={int(N)}|n_out={int(n_out)}"
+
+
+def lookup_dense(potential, N, n_out):
+    """(window_rows, graph) remembered for this problem, or None."""
+    k = key_dense(potential, N, n_out)
+    e = _MEM.get(k) or _load().get(k)
+    return None if e is None else (int(e["window_rows"]), bool(e["graph"]))
+
+
+def tune_dop853_dense(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e-10,
+                      order=None, mbs=CHUNK_MB, repeats=2, force=False, log=None):
+    """Pick (window_rows, graph) for gamd_dop853_dense_stream by timing whole
+    runs of this integration (wall clock: the call returns when the host
+    buffer is complete).  Returns dict(window_rows, graph, trials_ms)."""
+    import time
+    from .integrate import dop853_to_host
+    N = w0.shape[1]
+    n_out = len(t_out)
+    k = key_dense(potential, N, n_out)
+    if not force:
+        e = _MEM.get(k) or _load().get(k)
+        if e is not None:
+            return dict(e, cached=True)
+    t = dev.torch()
+    buf = t.empty((6, n_out, N), dtype=t.float64, pin_memory=True)
+    cands = sorted({chunk_rows_for(N, n_out, mb) for mb in mbs})
+    trials = {}
+    for rows in cands:
+        for graph in (False, True):
+            ts = []
+            for _ in range(repeats):
+                t_start = time.perf_counter()
+                dop853_to_host(potential, w0, t0, t1, t_out, h0=h0, rtol=rtol, atol=atol,
+                               order=order, window_rows=rows, graph=graph, out=buf)
+                ts.append((time.perf_counter() - t_start) * 1e3)
+            trials[f"window_rows={rows},graph={int(graph)}"] = ms = sum(ts) / len(ts)
+            if log:
+                log(f"autotune dop853 dense output: {rows} samples per window, graph "
+                    f"{int(graph)}: {ms:.1f} ms")
+    best = min(trials, key=trials.get)
+    rows = int(best.split(",")[0].split("=")[1])
+    graph = best.endswith("graph=1")
+    entry = {"window_rows": rows, "graph": graph, "trials_ms": trials}
     _store(k, entry)
     return dict(entry, cached=False)

@@ -59,6 +59,13 @@ struct DopArgs {
                            // wave on 16-lane rows; 2: the last <= 16 on 4-lane quads first
     double uround;         // the "step size too small" threshold (dop853.cpp:349-354)
     int64_t n_tail;        // queue slots [0, n_tail) go to dedicated wide waves
+    // dense output in windows of the t_out grid: w_out holds samples
+    // [out_base, out_base + out_rows), and a step samples up to out_end
+    // (the whole grid: 0, n_out, n_out).  pend: where a one-lane orbit keeps
+    // the polynomial of a step that also covers sample out_end (nullptr: no
+    // windows)
+    int64_t out_base, out_rows, out_end;
+    double *pend;
 };
 
 #define FOR6 _Pragma("unroll") for (int c = 0; c < 6; c++)
@@ -166,6 +173,7 @@ struct DopState {
     int64_t nstep, naccpt, nrejct, nfcn, oidx;
     int64_t stiff;  // naccpt % nstiff, kept incrementally (no 64-bit modulo per step)
     int reject, iasti, nonsti;
+    int pend;  // dense output in windows: the last step's polynomial is kept
 };
 
 // Lane layouts of one DOP853 system (the policy of dop_attempt):
@@ -178,6 +186,7 @@ struct DopState {
 template <bool SCF, int LAY>
 struct OneLane {
     static constexpr int NC = 6;
+    static constexpr bool WINDOWS = true;  // dense output in windows of the grid
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
         orbit_rhs<SCF, LAY>(P, y, f);
     }
@@ -192,8 +201,8 @@ struct OneLane {
     // dense sample s.oidx of orbit s.orb into w_out (6, n_out, N)
     __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
                                                 const double (&v)[6]) const {
-        const int64_t so = a.n_out * a.N;
-        FOR6 a.w_out[c * so + s.oidx * a.N + s.orb] = v[c];
+        const int64_t so = a.out_rows * a.N;
+        FOR6 a.w_out[c * so + (s.oidx - a.out_base) * a.N + s.orb] = v[c];
     }
 };
 
@@ -400,10 +409,10 @@ struct RowBodies {
     // dense sample s.oidx of body `body` into w_out (6, n_out, nb)
     __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
                                                 const double (&v)[2]) const {
-        const int64_t so = a.n_out * a.N;
+        const int64_t so = a.out_rows * a.N, row = (s.oidx - a.out_base) * a.N;
         if (body < nb && lane4 < 3) {
-            a.w_out[axis * so + s.oidx * a.N + body] = v[0];
-            a.w_out[(axis + 3) * so + s.oidx * a.N + body] = v[1];
+            a.w_out[axis * so + row + body] = v[0];
+            a.w_out[(axis + 3) * so + row + body] = v[1];
         }
     }
 };
@@ -455,8 +464,8 @@ struct WaveBodies {
     // dense sample s.oidx of body `body` into w_out (6, n_out, nb)
     __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
                                                 const double (&v)[6]) const {
-        const int64_t so = a.n_out * a.N;
-        if (body < nb) FOR6 a.w_out[c * so + s.oidx * a.N + body] = v[c];
+        const int64_t so = a.out_rows * a.N;
+        if (body < nb) FOR6 a.w_out[c * so + (s.oidx - a.out_base) * a.N + body] = v[c];
     }
 };
 
@@ -517,8 +526,8 @@ struct BatchLanes {
     // dense sample s.oidx of the lane's orbit into w_out (6, n_out, N)
     __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
                                                 const double (&v)[6]) const {
-        const int64_t so = a.n_out * a.N;
-        if (j < nb) FOR6 a.w_out[c * so + s.oidx * a.N + base + j] = v[c];
+        const int64_t so = a.out_rows * a.N;
+        if (j < nb) FOR6 a.w_out[c * so + (s.oidx - a.out_base) * a.N + base + j] = v[c];
     }
 };
 
@@ -559,6 +568,32 @@ __device__ double hinit_batch(const DevPot &P, const BatchLanes<SCF, LAY> &sys,
 }
 
 #define FORN _Pragma("unroll") for (int c = 0; c < NC; c++)
+
+// t_out sample `to` inside the step from x0 to x1 (either direction)
+__device__ __forceinline__ bool in_step(double x0, double x1, double to) {
+    return (x0 <= to && to <= x1) || (x1 <= to && to <= x0);
+}
+
+// the dense-output polynomial at `to` of the step (x0, h) (dop853.cpp:869-904)
+template <int NC>
+__device__ __forceinline__ void dense_value(double to, double x0, double h, const double *r1,
+                                            const double *r2, const double *r3, const double *r4,
+                                            const double *r5, const double *r6, const double *r7,
+                                            const double *r8, double (&v)[NC]) {
+    const double sv = (to - x0) / h, s1 = 1.0 - sv;
+    FORN v[c] = r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
+                sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
+}
+
+// layouts that support dense output in windows declare WINDOWS = true
+template <class S, class = void>
+struct has_windows {
+    static constexpr bool value = false;
+};
+template <class S>
+struct has_windows<S, decltype((void)S::WINDOWS)> {
+    static constexpr bool value = S::WINDOWS;
+};
 
 // One step attempt (dopcor basic step, dop853.cpp:339-648) for the lane's NC
 // components.  Returns 0 to go on, else the orbit's final idid.
@@ -712,17 +747,37 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                                  T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
                 }
                 const double x0 = s.x, x1 = s.x + h;
-                while (s.oidx < a.n_out) {
+                while (s.oidx < a.out_end) {
                     const double to = a.t_out[s.oidx];
-                    if ((x0 <= to && to <= x1) || (x1 <= to && to <= x0)) {
-                        const double sv = (to - s.x) / h, s1 = 1.0 - sv;
+                    if (in_step(x0, x1, to)) {
                         double v[NC];
-                        FORN v[c] = r1[c] + sv * (r2[c] + s1 * (r3[c] + sv * (r4[c] + s1 * (r5[c] +
-                                    sv * (r6[c] + s1 * (r7[c] + sv * r8[c]))))));
+                        dense_value<NC>(to, x0, h, r1, r2, r3, r4, r5, r6, r7, r8, v);
                         sys.store_dense(a, s, v);
                         s.oidx++;
                     } else {
                         break;
+                    }
+                }
+                if constexpr (has_windows<Sys>::value) {
+                    // the window ends inside this step: the next window
+                    // samples the rest of it from the same polynomial
+                    if (a.pend && s.oidx == a.out_end && s.oidx < a.n_out &&
+                        in_step(x0, x1, a.t_out[s.oidx])) {
+                        double *pp = a.pend + s.orb;
+                        const int64_t N = a.N;
+                        FORN {
+                            pp[(0 * NC + c) * N] = r1[c];
+                            pp[(1 * NC + c) * N] = r2[c];
+                            pp[(2 * NC + c) * N] = r3[c];
+                            pp[(3 * NC + c) * N] = r4[c];
+                            pp[(4 * NC + c) * N] = r5[c];
+                            pp[(5 * NC + c) * N] = r6[c];
+                            pp[(6 * NC + c) * N] = r7[c];
+                            pp[(7 * NC + c) * N] = r8[c];
+                        }
+                        pp[(8 * NC) * N] = x0;
+                        pp[(8 * NC + 1) * N] = h;
+                        s.pend = 1;
                     }
                 }
             }
@@ -913,7 +968,7 @@ __device__ __forceinline__ bool load_orbit(const DevPot &P, const DopArgs &a, in
     s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
     orbit_rhs<SCF, LAY>(P, y, k1);
     if (s.h == 0.0) s.h = hinit<SCF, LAY>(P, s.x, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
     return true;
@@ -932,7 +987,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
     s.facold = 1e-4;
     s.hlamb = 0.0;
     s.nstep = s.naccpt = s.stiff = s.nrejct = s.nfcn = s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
     double y[6], k1[6];
     FOR6 {
         y[c] = 0.0;
@@ -993,6 +1048,182 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         if (code != 0) {
             FOR6 a.w_final[c * a.N + s.orb] = y[c];
             finish_stats(a, s, code);
+            active = false;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense output in windows of the t_out grid (gamd_dop853_dense_stream).  The
+// launch for window [out_base, out_end) takes every orbit from where the
+// previous window left it, samples it into the window's slot and stops it at
+// the first step boundary past the window's last sample; its state is saved
+// and the slot is copied to host memory while the next window runs.  A step
+// that also covers the next window's first sample leaves its polynomial
+// (r1..r8, x0, h) behind, and the next window samples the rest of it before
+// the orbit steps on.  Steps, step sizes and samples are those of one
+// gamd_dop853 launch with w_out.
+//
+// Saved state, one row of N words per field:
+enum : int {
+    WS_X, WS_XEND, WS_H, WS_HMAX, WS_POSNEG, WS_FACOLD, WS_HLAMB,  // doubles
+    WS_NSTEP, WS_NACCPT, WS_NREJCT, WS_NFCN, WS_OIDX, WS_STIFF,    // int64
+    WS_REJECT, WS_IASTI, WS_NONSTI, WS_FLAGS,                       // int64
+    WS_Y, WS_K1 = WS_Y + 6, WS_WORDS = WS_K1 + 6
+};
+constexpr int WS_PEND_WORDS = 8 * 6 + 2;    // the kept step: r1..r8, x0, h
+constexpr int64_t WF_RUN = 1, WF_PEND = 2;  // still stepping; a kept step to sample
+
+struct WinArgs {
+    double *st;  // (WS_WORDS, N)
+    int first;   // the first window: orbits start from w0
+};
+
+__device__ __forceinline__ void win_save(const WinArgs &w, int64_t N, const DopState &s,
+                                         const double (&y)[6], const double (&k1)[6],
+                                         int64_t flags) {
+    double *d = w.st + s.orb;
+    int64_t *q = reinterpret_cast<int64_t *>(w.st) + s.orb;
+    d[WS_X * N] = s.x;
+    d[WS_XEND * N] = s.xend;
+    d[WS_H * N] = s.h;
+    d[WS_HMAX * N] = s.hmax;
+    d[WS_POSNEG * N] = s.posneg;
+    d[WS_FACOLD * N] = s.facold;
+    d[WS_HLAMB * N] = s.hlamb;
+    q[WS_NSTEP * N] = s.nstep;
+    q[WS_NACCPT * N] = s.naccpt;
+    q[WS_NREJCT * N] = s.nrejct;
+    q[WS_NFCN * N] = s.nfcn;
+    q[WS_OIDX * N] = s.oidx;
+    q[WS_STIFF * N] = s.stiff;
+    q[WS_REJECT * N] = s.reject;
+    q[WS_IASTI * N] = s.iasti;
+    q[WS_NONSTI * N] = s.nonsti;
+    q[WS_FLAGS * N] = flags;
+    FOR6 {
+        d[(WS_Y + c) * N] = y[c];
+        d[(WS_K1 + c) * N] = k1[c];
+    }
+}
+
+// the saved state of orbit orb; returns its flags
+__device__ __forceinline__ int64_t win_load(const WinArgs &w, int64_t N, int64_t orb,
+                                            DopState &s, double (&y)[6], double (&k1)[6]) {
+    const double *d = w.st + orb;
+    const int64_t *q = reinterpret_cast<const int64_t *>(w.st) + orb;
+    s.orb = orb;
+    s.x = d[WS_X * N];
+    s.xend = d[WS_XEND * N];
+    s.h = d[WS_H * N];
+    s.hmax = d[WS_HMAX * N];
+    s.posneg = d[WS_POSNEG * N];
+    s.facold = d[WS_FACOLD * N];
+    s.hlamb = d[WS_HLAMB * N];
+    s.nstep = q[WS_NSTEP * N];
+    s.naccpt = q[WS_NACCPT * N];
+    s.nrejct = q[WS_NREJCT * N];
+    s.nfcn = q[WS_NFCN * N];
+    s.oidx = q[WS_OIDX * N];
+    s.stiff = q[WS_STIFF * N];
+    s.reject = (int)q[WS_REJECT * N];
+    s.iasti = (int)q[WS_IASTI * N];
+    s.nonsti = (int)q[WS_NONSTI * N];
+    s.pend = 0;
+    FOR6 {
+        y[c] = d[(WS_Y + c) * N];
+        k1[c] = d[(WS_K1 + c) * N];
+    }
+    return q[WS_FLAGS * N];
+}
+
+// samples [s.oidx, out_end) of the step kept by the previous window; returns
+// true when the step still covers the sample after this window's last
+template <bool SCF, int LAY>
+__device__ __forceinline__ bool sample_kept(const DopArgs &a, DopState &s) {
+    constexpr int NC = 6;
+    const double *pp = a.pend + s.orb;
+    const int64_t N = a.N;
+    double r[8][NC];
+#pragma unroll
+    for (int k = 0; k < 8; k++) FORN r[k][c] = pp[(k * NC + c) * N];
+    const double x0 = pp[(8 * NC) * N], h = pp[(8 * NC + 1) * N], x1 = x0 + h;
+    const OneLane<SCF, LAY> sys;
+    while (s.oidx < a.out_end) {
+        const double to = a.t_out[s.oidx];
+        if (!in_step(x0, x1, to)) return false;
+        double v[NC];
+        dense_value<NC>(to, x0, h, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], v);
+        sys.store_dense(a, s, v);
+        s.oidx++;
+    }
+    return s.oidx < a.n_out && in_step(x0, x1, a.t_out[s.oidx]);
+}
+
+// one window: the persistent one-lane queue of k_dop853 over every orbit
+template <bool SCF, int LAY>
+__global__ __launch_bounds__(256) void k_dop853_window(const DevPot *__restrict__ Pp, DopArgs a,
+                                                       WinArgs w) {
+    const DevPot &P = *Pp;
+    const int lane = threadIdx.x & 63;
+    const bool last_window = a.out_end >= a.n_out;
+    bool active = false, exhausted = false;
+    DopState s;
+    s.orb = 0;
+    s.x = s.xend = s.h = s.hmax = 0.0;
+    s.posneg = 1.0;
+    s.facold = 1e-4;
+    s.hlamb = 0.0;
+    s.nstep = s.naccpt = s.stiff = s.nrejct = s.nfcn = s.oidx = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
+    double y[6], k1[6];
+    FOR6 {
+        y[c] = 0.0;
+        k1[c] = 0.0;
+    }
+    while (true) {
+        const bool need = !active && !exhausted;
+        const unsigned long long want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((long long)want) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(want));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned long long below = want & ((1ull << lane) - 1ull);
+                const unsigned long long slot = base + (unsigned long long)__popcll(below);
+                if (slot < (unsigned long long)a.N) {
+                    if (w.first) {
+                        active = load_orbit<SCF, LAY>(P, a, (int64_t)slot, s, y, k1);
+                    } else {
+                        const int64_t orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
+                        // slots that name no orbit were skipped by the first window too
+                        int64_t flags = 0;
+                        if (orb >= 0 && orb < a.N) flags = win_load(w, a.N, orb, s, y, k1);
+                        if (flags & WF_PEND) {
+                            const bool more = sample_kept<SCF, LAY>(a, s);
+                            flags = (flags & WF_RUN) | (more ? WF_PEND : 0);
+                            if (more || !(flags & WF_RUN)) win_save(w, a.N, s, y, k1, flags);
+                            if (more) flags = 0;  // no steps before its samples are out
+                        }
+                        active = (flags & WF_RUN) != 0;
+                    }
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (!active) continue;
+        const int code = dop_attempt<SCF, true, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
+        if (code != 0) {
+            FOR6 a.w_final[c * a.N + s.orb] = y[c];
+            finish_stats(a, s, code);
+            if (!last_window) win_save(w, a.N, s, y, k1, s.pend ? WF_PEND : 0);
+            active = false;
+        } else if (!last_window && s.oidx >= a.out_end) {
+            // every sample before the window's end is out: stop at this step
+            win_save(w, a.N, s, y, k1, WF_RUN | (s.pend ? WF_PEND : 0));
             active = false;
         }
     }
@@ -1085,7 +1316,7 @@ __global__ __launch_bounds__(256) void k_dop853_stream(const DevPot *__restrict_
     s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
     sys.rhs(P, y, k1);
     int code = 0;
     // every group ends: each attempt advances nstep, and nmax bounds it
@@ -1155,7 +1386,7 @@ __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ 
     s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
     sys.rhs(P, y, k1);
     int code = 0;
     // the group ends: each attempt advances nstep, and nmax bounds it
@@ -1203,7 +1434,7 @@ __global__ __launch_bounds__(MAXT) void k_dop853_batch(const DevPot *__restrict_
     s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
     s.nfcn = 2;
     s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = 0;
+    s.reject = s.iasti = s.nonsti = s.pend = 0;
     sys.rhs(P, y, k1);
     if (s.h == 0.0) s.h = hinit_batch<SCF, LAY>(P, sys, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
     int code = 0;
@@ -1294,17 +1525,12 @@ int dop853_check(const gamd_potential *pot, const double *w0, const double *t0, 
     return GAMD_OK;
 }
 
-// The launches of gamd_dop853 after its checks, with the work-queue counter
-// in `queue` (device memory the caller provides; cleared here by a kernel,
-// so that the sequence holds no memset node).
-int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
-               const double *t1, const double *h0, double rtol, double atol, int64_t nmax,
-               int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
-               double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
-               int64_t n_tail, int block, unsigned long long *queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, queue);
-    int rc = launch_status();
-    if (rc != GAMD_OK) return rc;
+// gamd_dop853's kernel arguments
+DopArgs dop_args(const double *w0, size_t N, const double *t0, const double *t1, const double *h0,
+                 double rtol, double atol, int64_t nmax, int64_t nstiff, double hmax,
+                 const double *t_out, size_t n_out, double *w_out, double *w_final,
+                 int32_t *status, int64_t *stats, const int32_t *order, int64_t n_tail,
+                 unsigned long long *queue) {
     DopArgs a;
     a.w0 = w0;
     a.N = (int64_t)N;
@@ -1318,6 +1544,9 @@ int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const doub
     a.hmax = hmax;
     a.t_out = t_out;
     a.n_out = (int64_t)n_out;
+    a.out_base = 0;
+    a.out_rows = a.out_end = a.n_out;
+    a.pend = nullptr;
     a.w_out = w_out;
     a.w_final = w_final;
     a.status = status;
@@ -1325,6 +1554,27 @@ int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const doub
     a.order = order;
     a.n_tail = n_tail;
     a.queue = queue;
+    // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end, =1 skips the
+    // quad stage (same results every way)
+    const char *wide_env = getenv("GAMD_DOP853_WIDE");
+    a.wide = !wide_env ? 2 : (wide_env[0] == '0' ? 0 : (wide_env[0] == '1' ? 1 : 2));
+    a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
+    return a;
+}
+
+// The launches of gamd_dop853 after its checks, with the work-queue counter
+// in `queue` (device memory the caller provides; cleared here by a kernel,
+// so that the sequence holds no memset node).
+int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const double *t0,
+               const double *t1, const double *h0, double rtol, double atol, int64_t nmax,
+               int64_t nstiff, double hmax, const double *t_out, size_t n_out, double *w_out,
+               double *w_final, int32_t *status, int64_t *stats, const int32_t *order,
+               int64_t n_tail, int block, unsigned long long *queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, queue);
+    int rc = launch_status();
+    if (rc != GAMD_OK) return rc;
+    DopArgs a = dop_args(w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, w_out,
+                         w_final, status, stats, order, n_tail, queue);
     if (order && status) {
         // an orbit that the order does not name keeps this status
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, status,
@@ -1332,11 +1582,6 @@ int dop853_run(const gamd_potential *pot, const double *w0, size_t N, const doub
         rc = launch_status();
         if (rc != GAMD_OK) return rc;
     }
-    // GAMD_DOP853_WIDE=0 keeps one lane per orbit to the end, =1 skips the
-    // quad stage (same results every way)
-    const char *wide_env = getenv("GAMD_DOP853_WIDE");
-    a.wide = !wide_env ? 2 : (wide_env[0] == '0' ? 0 : (wide_env[0] == '1' ? 1 : 2));
-    a.uround = 2.220446049250313e-16;  // np.finfo(float).eps, dop853.pyx:165
     const bool scf = pot->dev.has_scf;
     const bool mw = pot->dev.layout == LAYOUT_MW;
     if (w_out)
@@ -1423,6 +1668,164 @@ extern "C" int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, 
     // the counter is the array's first word; the kernels read no other
     return dop853_run(pot, w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, w_out,
                       w_final, status, stats, order, n_tail, block, queue, as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// gamd_dop853_dense_stream: gamd_dop853's dense output in windows of
+// `window_rows` samples, each window's slot copied to host memory on a
+// second stream while the next window runs (two slots in turn).
+
+namespace {
+
+// the window kernel's persistent grid: every workgroup that fits at once,
+// at most one lane per orbit (queried before any capture begins)
+template <bool SCF, int LAY>
+int64_t window_grid(int block, int64_t N) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_window<SCF, LAY>, block,
+                                                     0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    int64_t blocks = (int64_t)per_cu * cu_count();
+    const int64_t needed = (N + block - 1) / block;
+    if (blocks > needed) blocks = needed;
+    return blocks < 1 ? 1 : blocks;
+}
+
+// the window launches and their copies; kernels on s, copies on c
+template <bool SCF, int LAY>
+int dense_windows(const DevPot *P, DopArgs a, int block, int64_t blocks, int64_t W,
+                  double *host_out, double *ring, WinArgs w, hipStream_t s, hipStream_t c,
+                  hipEvent_t comp, hipEvent_t copied[2]) {
+    auto kern = k_dop853_window<SCF, LAY>;
+    const int64_t n_out = a.n_out, N = a.N;
+    const int64_t nwin = (n_out + W - 1) / W;
+    if (a.order && a.status) {
+        // an orbit that the order does not name keeps this status
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+                           a.status, N, (int32_t)GAMD_STATUS_NOT_RUN);
+        const int rc = launch_status();
+        if (rc != GAMD_OK) return rc;
+    }
+    for (int64_t k = 0; k < nwin; k++) {
+        const int sl = (int)(k & 1);
+        double *slot = ring + (size_t)sl * 6 * W * N;
+        if (k >= 2) GAMD_HIP_CHECK(hipStreamWaitEvent(s, copied[sl], 0));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, a.queue);
+        int rc = launch_status();
+        if (rc != GAMD_OK) return rc;
+        DopArgs b = a;
+        b.w_out = slot;
+        b.out_base = k * W;
+        b.out_rows = W;
+        b.out_end = (k + 1) * W < n_out ? (k + 1) * W : n_out;
+        WinArgs wk = w;
+        wk.first = k == 0;
+        hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, s, P, b, wk);
+        rc = launch_status();
+        if (rc != GAMD_OK) return rc;
+        GAMD_HIP_CHECK(hipEventRecord(comp, s));
+        GAMD_HIP_CHECK(hipStreamWaitEvent(c, comp, 0));
+        const int64_t rows = b.out_end - b.out_base;
+        for (int q = 0; q < 6; q++)
+            GAMD_HIP_CHECK(hipMemcpyAsync(host_out + ((size_t)q * n_out + b.out_base) * N,
+                                          slot + (size_t)q * W * N,
+                                          (size_t)rows * N * sizeof(double),
+                                          hipMemcpyDeviceToHost, c));
+        GAMD_HIP_CHECK(hipEventRecord(copied[sl], c));
+    }
+    // join: the compute stream waits for the last copies
+    GAMD_HIP_CHECK(hipStreamWaitEvent(s, copied[(nwin - 1) & 1], 0));
+    if (nwin >= 2) GAMD_HIP_CHECK(hipStreamWaitEvent(s, copied[(nwin - 2) & 1], 0));
+    return GAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t gamd_dop853_dense_workspace_bytes(size_t N, int64_t window_rows) {
+    if (window_rows < 1) return -1;
+    return (int64_t)(256 + (2 * 6 * (size_t)window_rows + WS_WORDS + WS_PEND_WORDS) * N *
+                               sizeof(double));
+}
+
+extern "C" int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t N,
+                                        const double *t0, const double *t1, const double *h0,
+                                        double rtol, double atol, int64_t nmax, int64_t nstiff,
+                                        double hmax, const double *t_out, size_t n_out,
+                                        int64_t window_rows, int use_graph, double *host_out,
+                                        double *w_final, int32_t *status, int64_t *stats,
+                                        const int32_t *order, int block, void *workspace,
+                                        size_t workspace_bytes, void *stream) {
+    int rc = dop853_check(pot, w0, t0, t1, w_final, host_out, t_out, n_out, 0, block, nmax,
+                          nstiff);
+    if (rc != GAMD_OK) return rc;
+    if (!host_out || !t_out || n_out == 0 || window_rows < 1 || !workspace)
+        return GAMD_ERR_INVALID;
+    if ((int64_t)workspace_bytes < gamd_dop853_dense_workspace_bytes(N, window_rows))
+        return GAMD_ERR_INVALID;
+    if (N == 0) return GAMD_OK;
+    const int64_t W = window_rows < (int64_t)n_out ? window_rows : (int64_t)n_out;
+    unsigned long long *queue = static_cast<unsigned long long *>(workspace);
+    double *ring = reinterpret_cast<double *>(static_cast<char *>(workspace) + 256);
+    double *st = ring + 2 * 6 * (size_t)window_rows * N;
+    DopArgs a = dop_args(w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, nullptr,
+                         w_final, status, stats, order, 0, queue);
+    a.wide = 0;
+    a.pend = st + (size_t)WS_WORDS * N;
+    WinArgs w;
+    w.st = st;
+    w.first = 1;
+    hipStream_t s = as_stream(stream);
+    hipStream_t c = nullptr, cs = nullptr;
+    hipEvent_t comp = nullptr, copied[2] = {nullptr, nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const bool scf = pot->dev.has_scf;
+    const bool mw = pot->dev.layout == LAYOUT_MW;
+    auto run = scf  ? dense_windows<true, LAYOUT_GENERIC>
+               : mw ? dense_windows<false, LAYOUT_MW>
+                    : dense_windows<false, LAYOUT_GENERIC>;
+    const int64_t blocks = scf  ? window_grid<true, LAYOUT_GENERIC>(block, (int64_t)N)
+                           : mw ? window_grid<false, LAYOUT_MW>(block, (int64_t)N)
+                                : window_grid<false, LAYOUT_GENERIC>(block, (int64_t)N);
+    rc = GAMD_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c, hipStreamNonBlocking) != hipSuccess) goto done;
+    if (hipEventCreateWithFlags(&comp, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&copied[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&copied[1], hipEventDisableTiming) != hipSuccess)
+        goto done;
+    if (!use_graph) {
+        rc = run(pot->ddev, a, block, blocks, W, host_out, ring, w, s, c, comp, copied);
+        // the events and the copy stream are released below: finish first
+        if (rc == GAMD_OK && hipStreamSynchronize(s) != hipSuccess) rc = GAMD_ERR_HIP;
+    } else {
+        // the same sequence captured once on a private stream (the caller's
+        // may be the legacy default stream, which cannot capture): kernel and
+        // copy nodes, no memset; then launched as one graph on the caller's
+        // stream, after its earlier work
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) goto done;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) goto done;
+        rc = run(pot->ddev, a, block, blocks, W, host_out, ring, w, cs, c, comp, copied);
+        const hipError_t ce = hipStreamEndCapture(cs, &graph);
+        if (rc != GAMD_OK || ce != hipSuccess || !graph) {
+            rc = GAMD_ERR_HIP;
+            goto done;
+        }
+        rc = GAMD_ERR_HIP;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) goto done;
+        if (hipGraphLaunch(exec, s) != hipSuccess) goto done;
+        if (hipStreamSynchronize(s) != hipSuccess) goto done;
+        rc = GAMD_OK;
+    }
+done:
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (comp) (void)hipEventDestroy(comp);
+    if (copied[0]) (void)hipEventDestroy(copied[0]);
+    if (copied[1]) (void)hipEventDestroy(copied[1]);
+    if (c) (void)hipStreamDestroy(c);
+    if (cs) (void)hipStreamDestroy(cs);
+    return rc;
 }
 
 extern "C" int gamd_mockstream_dop853(const gamd_potential *ext,
@@ -1522,6 +1925,7 @@ extern "C" int gamd_nbody_dop853(const gamd_potential *ext, const gamd_potential
     a.hmax = hmax;
     a.t_out = t_out;
     a.n_out = w_out ? (int64_t)n_out : 0;
+    a.out_rows = a.out_end = a.n_out;
     a.w_out = w_out;
     a.w_final = w_final;
     a.status = status;
@@ -1573,6 +1977,7 @@ extern "C" int gamd_dop853_batched(const gamd_potential *pot, const double *w0, 
     a.hmax = hmax;
     a.t_out = t_out;
     a.n_out = w_out ? (int64_t)n_out : 0;
+    a.out_rows = a.out_end = a.n_out;
     a.w_out = w_out;
     a.w_final = w_final;
     a.status = status;

@@ -14,8 +14,8 @@ from . import _lib
 
 __all__ = ["parse_time_specification", "get_integrator", "LeapfrogIntegrator",
            "DOPRI853Integrator", "Ruth4Integrator", "RK5Integrator", "leapfrog", "ruth4", "dop853",
-           "dop853_batched", "leapfrog_to_host",
-           "pilot_order", "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
+           "dop853_batched", "leapfrog_to_host", "dop853_to_host", "pilot_order",
+           "nbody_leapfrog", "nbody_ruth4", "nbody_dop853", "integrate_hamiltonian"]
 
 
 class Integrator:
@@ -245,6 +245,65 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
     return out
 
 
+def dop853_to_host(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e-10, nmax=0,
+                   nstiff=0, hmax=0.0, order=None, block=0, window_rows=None, graph=None,
+                   out=None):
+    """dop853(..., t_out=t_out) with the dense output streamed to pinned host
+    memory while the integration runs (gamd_dop853_dense_stream): the t_out
+    grid is taken in windows of window_rows samples, each window's samples
+    copied on a second stream while the next window integrates.  window_rows
+    / graph: None = the autotuner's remembered choice for this problem, else
+    a default of ~16 windows.  order: a permutation or "pilot" (as dop853).
+    out: a pinned host float64 tensor (6, n_out, N) to reuse.  Returns
+    dict(w_out = numpy view of the pinned buffer (6, n_out, N), w_final,
+    status, stats (device)), equal bit for bit to dop853's."""
+    from . import autotune
+    N = dev.check_w0(w0)
+    for v, name in ((t0, "t0"), (t1, "t1"), (h0, "h0")):
+        dev.check_per_orbit(v, N, name)
+    if order is not None and not isinstance(order, str):
+        dev.check_permutation(order, N)
+    t = dev.require_gpu()
+    lib = _lib.load()
+    w0 = dev.phase_space(w0)
+    if isinstance(order, str):
+        if order != "pilot":
+            raise ValueError(f"unknown order {order!r}")
+        order = pilot_order(potential, w0, t0, t1, h0, rtol=rtol, atol=atol, nmax=nmax,
+                            nstiff=nstiff, hmax=hmax, block=block)
+    tout = dev.to_dev(t_out)
+    if tout.ndim != 1 or tout.shape[0] < 1:
+        raise ValueError("t_out must be a non-empty 1-d array of times")
+    n_out = int(tout.shape[0])
+    if window_rows is None or graph is None:
+        tuned = autotune.lookup_dense(potential, N, n_out)
+        if window_rows is None:
+            window_rows = tuned[0] if tuned else autotune.default_chunk_rows(N, n_out)
+        if graph is None:
+            graph = tuned[1] if tuned else False
+    window_rows = max(1, min(int(window_rows), n_out))
+    t0d, t1d = dev.per_orbit(t0, N, "t0"), dev.per_orbit(t1, N, "t1")
+    h0d = dev.per_orbit(h0, N, "h0")
+    if out is None or tuple(out.shape) != (6, n_out, N):
+        out = t.empty((6, n_out, N), dtype=t.float64, pin_memory=True)
+    wfin = dev.empty((6, N))
+    status = dev.empty((N,), dtype=t.int32)
+    stats = dev.empty((N, 4), dtype=t.int64)
+    ordd = dev.to_dev(order, dtype=t.int32) if order is not None else None  # checked above
+    nbytes = int(lib.gamd_dop853_dense_workspace_bytes(N, window_rows))
+    ws = t.empty(max(nbytes, 8), dtype=t.uint8, device="cuda")
+    _lib.check(lib.gamd_dop853_dense_stream(potential.c_handle, w0.data_ptr(), N,
+                                            t0d.data_ptr(), t1d.data_ptr(), dev.ptr(h0d),
+                                            float(rtol), float(atol), int(nmax), int(nstiff),
+                                            float(hmax), tout.data_ptr(), n_out, window_rows,
+                                            int(bool(graph)), out.data_ptr(), wfin.data_ptr(),
+                                            status.data_ptr(), stats.data_ptr(), dev.ptr(ordd),
+                                            int(block), ws.data_ptr(), nbytes,
+                                            dev.stream_ptr()),
+               "gamd_dop853_dense_stream")
+    return {"w_out": out.numpy(), "w_final": wfin, "status": status, "stats": stats}
+
+
 MAX_NBATCH = 1024  # GAMD_DOP853_MAX_NBATCH: orbits of one batched system
 
 
@@ -417,7 +476,11 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
                   hmax=Integrator_kwargs.get("dt_max", 0.0),
                   t_out=t if save_all else None)
         nbatch = int(Integrator_kwargs.get("nbatch", 100))  # chamiltonian.pyx:348
-        if nbatch == 1:
+        if nbatch == 1 and save_all:
+            # the dense output streams to pinned host memory while it is computed
+            kw.pop("t_out")
+            res = dop853_to_host(pot, arr, t[0], t[-1], t, h0=t[1] - t[0], **kw)
+        elif nbatch == 1:
             res = dop853(pot, arr, t[0], t[-1], t[1] - t[0], **kw)
         else:
             res = dop853_batched(pot, arr, t[0], t[-1], t[1] - t[0], nbatch=nbatch, **kw)
@@ -425,7 +488,10 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
         if int(Integrator_kwargs.get("err_if_fail", 1)) == 1 and np.any(status < 0):
             bad = int(status[status < 0][0])
             raise RuntimeError(f"Integration failed with code {bad}")
-        w = dev.to_host(res["w_out"]) if save_all else dev.to_host(res["w_final"])
+        if save_all:
+            w = res["w_out"] if nbatch == 1 else dev.to_host(res["w_out"])
+        else:
+            w = dev.to_host(res["w_final"])
     else:
         raise ValueError(f"Cython integration not supported for '{Integrator!r}'")
     if w.shape[-1] == 1:  # chamiltonian.pyx:355-356

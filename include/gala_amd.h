@@ -204,6 +204,36 @@ int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, size_t N,
                         int32_t *status, int64_t *stats, const int32_t *order, int64_t n_tail,
                         int block, void *stream);
 
+/* gamd_dop853's dense output streamed to HOST memory while the integration
+ * runs (gala's dop853_integrate_hamiltonian returns its dense output on the
+ * host, dop853.pyx:228-250; the samples are those of dop853.cpp:584-612,
+ * 869-904).  The n_out shared times t_out (device) are taken in windows of
+ * window_rows samples: one launch per window carries every orbit to the
+ * first step boundary past the window's last sample (its state kept in the
+ * workspace between windows; a step that also covers the next window's first
+ * sample keeps its dense-output polynomial for that window) and writes the
+ * window's samples to a slot of a two-slot device ring, copied to host_out
+ * (HOST, ideally pinned, (6, n_out, N)) on a second stream while the next
+ * window runs.  Steps, samples, w_final, status and stats equal gamd_dop853's
+ * with w_out bit for bit; a sample outside an orbit's [t0, t1] is not
+ * written (gamd_dop853 leaves it unwritten too).  Orbits run one per lane
+ * (no n_tail rows, as gamd_dop853 with dense output).  use_graph != 0
+ * captures the window kernels and copies as one HIP graph and launches that
+ * (no memset nodes).  workspace: device,
+ * gamd_dop853_dense_workspace_bytes(N, window_rows) bytes (the work-queue
+ * counter included: the call allocates no device memory).  Returns when
+ * host_out is complete (it synchronises `stream`).  Other arguments as
+ * gamd_dop853.  No gala counterpart for the windows themselves; window_rows
+ * is a speed knob (gala_amd.autotune picks it). */
+int64_t gamd_dop853_dense_workspace_bytes(size_t N, int64_t window_rows);
+int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t N,
+                             const double *t0, const double *t1, const double *h0, double rtol,
+                             double atol, int64_t nmax, int64_t nstiff, double hmax,
+                             const double *t_out, size_t n_out, int64_t window_rows,
+                             int use_graph, double *host_out, double *w_final, int32_t *status,
+                             int64_t *stats, const int32_t *order, int block, void *workspace,
+                             size_t workspace_bytes, void *stream);
+
 #define GAMD_DOP853_MAX_NBATCH 1024  /* orbits of one batched system (one workgroup) */
 
 /* Replaces dop853_integrate_hamiltonian with gala's default batching
