@@ -67,8 +67,8 @@ SIGNATURES = {
                                    _sz, _vp, _vp, _vp, _vp, _vp, _i64, _int, _vp]),
     "gamd_dop853_dense_workspace_bytes": (_i64, [_sz, _i64]),
     "gamd_dop853_dense_stream": (_int, [_vp, _vp, _sz, _vp, _vp, _vp, _d, _d, _i64, _i64, _d,
-                                        _vp, _sz, _i64, _int, _vp, _vp, _vp, _vp, _vp, _int,
-                                        _vp, _sz, _vp]),
+                                        _vp, _sz, _i64, _int, _vp, _vp, _vp, _vp, _vp, _i64,
+                                        _int, _vp, _sz, _vp]),
     "gamd_dop853_batched": (_int, [_vp, _vp, _sz, _d, _d, _d, _d, _d, _i64, _i64, _d, _int, _vp,
                                    _sz, _vp, _vp, _vp, _vp, _vp]),
     "gamd_nbody_leapfrog": (_int, [_vp, _vp, _int, _vp, _sz, _d, _d, _i64, _i64, _vp, _vp,

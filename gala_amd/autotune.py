@@ -223,7 +223,7 @@ def lookup_dense(potential, N, n_out):
 
 
 def tune_dop853_dense(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e-10,
-                      order=None, mbs=CHUNK_MB, repeats=2, force=False, log=None):
+                      order=None, n_tail=0, mbs=CHUNK_MB, repeats=2, force=False, log=None):
     """Pick (window_rows, graph) for gamd_dop853_dense_stream by timing whole
     runs of this integration (wall clock: the call returns when the host
     buffer is complete).  Returns dict(window_rows, graph, trials_ms)."""
@@ -246,7 +246,8 @@ def tune_dop853_dense(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e
             for _ in range(repeats):
                 t_start = time.perf_counter()
                 dop853_to_host(potential, w0, t0, t1, t_out, h0=h0, rtol=rtol, atol=atol,
-                               order=order, window_rows=rows, graph=graph, out=buf)
+                               order=order, n_tail=n_tail, window_rows=rows, graph=graph,
+                               out=buf)
                 ts.append((time.perf_counter() - t_start) * 1e3)
             trials[f"window_rows={rows},graph={int(graph)}"] = ms = sum(ts) / len(ts)
             if log:

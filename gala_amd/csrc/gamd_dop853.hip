@@ -176,6 +176,112 @@ struct DopState {
     int pend;  // dense output in windows: the last step's polynomial is kept
 };
 
+// Dense output in windows of the t_out grid (gamd_dop853_dense_stream).  The
+// launch for window [out_base, out_end) takes every orbit from where the
+// previous window left it, samples it into the window's slot and stops it at
+// the first step boundary past the window's last sample; its state is kept
+// and the slot is copied to host memory while the next window runs.  A step
+// that also covers the next window's first sample keeps its polynomial
+// (r1..r8, x0, h), and the next window samples the rest of it before the
+// orbit steps on.  Steps, step sizes and samples are those of one gamd_dop853
+// launch with w_out.
+//
+// Kept state, one row of N words per field:
+enum : int {
+    WS_X, WS_XEND, WS_H, WS_HMAX, WS_POSNEG, WS_FACOLD, WS_HLAMB,  // doubles
+    WS_NSTEP, WS_NACCPT, WS_NREJCT, WS_NFCN, WS_OIDX, WS_STIFF,    // int64
+    WS_REJECT, WS_IASTI, WS_NONSTI, WS_FLAGS,                       // int64
+    WS_Y, WS_K1 = WS_Y + 6, WS_WORDS = WS_K1 + 6
+};
+constexpr int WS_PEND_WORDS = 8 * 6 + 2;    // the kept step: r1..r8, x0, h
+constexpr int64_t WF_RUN = 1, WF_PEND = 2;  // still stepping; a kept step to sample
+
+struct WinArgs {
+    double *st;  // (WS_WORDS, N); nullptr: no windows
+    int first;   // the first window: orbits start from w0
+};
+
+__device__ __forceinline__ void win_save_scalars(const WinArgs &w, int64_t N, const DopState &s,
+                                                 int64_t flags) {
+    double *d = w.st + s.orb;
+    int64_t *q = reinterpret_cast<int64_t *>(w.st) + s.orb;
+    d[WS_X * N] = s.x;
+    d[WS_XEND * N] = s.xend;
+    d[WS_H * N] = s.h;
+    d[WS_HMAX * N] = s.hmax;
+    d[WS_POSNEG * N] = s.posneg;
+    d[WS_FACOLD * N] = s.facold;
+    d[WS_HLAMB * N] = s.hlamb;
+    q[WS_NSTEP * N] = s.nstep;
+    q[WS_NACCPT * N] = s.naccpt;
+    q[WS_NREJCT * N] = s.nrejct;
+    q[WS_NFCN * N] = s.nfcn;
+    q[WS_OIDX * N] = s.oidx;
+    q[WS_STIFF * N] = s.stiff;
+    q[WS_REJECT * N] = s.reject;
+    q[WS_IASTI * N] = s.iasti;
+    q[WS_NONSTI * N] = s.nonsti;
+    q[WS_FLAGS * N] = flags;
+}
+
+// component c (0..5) of the state and of its first derivative
+__device__ __forceinline__ void win_save_comp(const WinArgs &w, int64_t N, int64_t orb, int c,
+                                              double y, double k1) {
+    w.st[(WS_Y + c) * N + orb] = y;
+    w.st[(WS_K1 + c) * N + orb] = k1;
+}
+
+// the kept state of orbit orb into one lane; returns its flags
+__device__ __forceinline__ int64_t win_load(const WinArgs &w, int64_t N, int64_t orb,
+                                            DopState &s, double (&y)[6], double (&k1)[6]) {
+    const double *d = w.st + orb;
+    const int64_t *q = reinterpret_cast<const int64_t *>(w.st) + orb;
+    s.orb = orb;
+    s.x = d[WS_X * N];
+    s.xend = d[WS_XEND * N];
+    s.h = d[WS_H * N];
+    s.hmax = d[WS_HMAX * N];
+    s.posneg = d[WS_POSNEG * N];
+    s.facold = d[WS_FACOLD * N];
+    s.hlamb = d[WS_HLAMB * N];
+    s.nstep = q[WS_NSTEP * N];
+    s.naccpt = q[WS_NACCPT * N];
+    s.nrejct = q[WS_NREJCT * N];
+    s.nfcn = q[WS_NFCN * N];
+    s.oidx = q[WS_OIDX * N];
+    s.stiff = q[WS_STIFF * N];
+    s.reject = (int)q[WS_REJECT * N];
+    s.iasti = (int)q[WS_IASTI * N];
+    s.nonsti = (int)q[WS_NONSTI * N];
+    s.pend = 0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        y[c] = d[(WS_Y + c) * N];
+        k1[c] = d[(WS_K1 + c) * N];
+    }
+    return q[WS_FLAGS * N];
+}
+
+// component c of a kept step's polynomial, and the step's x0 and h
+__device__ __forceinline__ void keep_comp(const DopArgs &a, int64_t orb, int c, double r1,
+                                          double r2, double r3, double r4, double r5, double r6,
+                                          double r7, double r8) {
+    double *pp = a.pend + orb;
+    const int64_t N = a.N;
+    pp[(0 * 6 + c) * N] = r1;
+    pp[(1 * 6 + c) * N] = r2;
+    pp[(2 * 6 + c) * N] = r3;
+    pp[(3 * 6 + c) * N] = r4;
+    pp[(4 * 6 + c) * N] = r5;
+    pp[(5 * 6 + c) * N] = r6;
+    pp[(6 * 6 + c) * N] = r7;
+    pp[(7 * 6 + c) * N] = r8;
+}
+__device__ __forceinline__ void keep_x0h(const DopArgs &a, int64_t orb, double x0, double h) {
+    a.pend[(8 * 6) * a.N + orb] = x0;
+    a.pend[(8 * 6 + 1) * a.N + orb] = h;
+}
+
 // Lane layouts of one DOP853 system (the policy of dop_attempt):
 //   rhs   the lane's share of f(y);
 //   sums  sa += a_i^2, sb += b_i^2 over the system's n components in the
@@ -198,11 +304,27 @@ struct OneLane {
         }
     }
     __device__ __forceinline__ double n() const { return 6.0; }
-    // dense sample s.oidx of orbit s.orb into w_out (6, n_out, N)
+    // dense sample s.oidx of orbit s.orb into w_out (6, out_rows, N)
     __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
                                                 const double (&v)[6]) const {
         const int64_t so = a.out_rows * a.N;
         FOR6 a.w_out[c * so + (s.oidx - a.out_base) * a.N + s.orb] = v[c];
+    }
+    // the step that covers the next window's first sample
+    __device__ __forceinline__ void keep_step(const DopArgs &a, const DopState &s,
+                                              const double (&r1)[6], const double (&r2)[6],
+                                              const double (&r3)[6], const double (&r4)[6],
+                                              const double (&r5)[6], const double (&r6)[6],
+                                              const double (&r7)[6], const double (&r8)[6],
+                                              double x0, double h) const {
+        FOR6 keep_comp(a, s.orb, c, r1[c], r2[c], r3[c], r4[c], r5[c], r6[c], r7[c], r8[c]);
+        keep_x0h(a, s.orb, x0, h);
+    }
+    __device__ __forceinline__ void save_state(const WinArgs &w, int64_t N, const DopState &s,
+                                               const double (&y)[6], const double (&k1)[6],
+                                               int64_t flags) const {
+        win_save_scalars(w, N, s, flags);
+        FOR6 win_save_comp(w, N, s.orb, c, y[c], k1[c]);
     }
 };
 
@@ -211,6 +333,7 @@ template <bool SCF, int LAY>
 struct WideRow {
     static constexpr int NC = 1;
     int comp;
+    bool writes;  // lanes 0..5 of the row store the dense samples
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[1], double (&f)[1]) const {
         const double qx = row_bcast<0>(y[0]), qy = row_bcast<1>(y[0]), qz = row_bcast<2>(y[0]);
         const double vp = row_shl3(y[0]);  // the velocity partner of components 0..2
@@ -238,6 +361,31 @@ struct WideRow {
         }
     }
     __device__ __forceinline__ double n() const { return 6.0; }
+    // dense sample s.oidx of orbit s.orb into w_out (6, out_rows, N)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[1]) const {
+        if (writes) a.w_out[comp * a.out_rows * a.N + (s.oidx - a.out_base) * a.N + s.orb] = v[0];
+    }
+    static constexpr bool WINDOWS = true;
+    __device__ __forceinline__ void keep_step(const DopArgs &a, const DopState &s,
+                                              const double (&r1)[1], const double (&r2)[1],
+                                              const double (&r3)[1], const double (&r4)[1],
+                                              const double (&r5)[1], const double (&r6)[1],
+                                              const double (&r7)[1], const double (&r8)[1],
+                                              double x0, double h) const {
+        if (writes) {
+            keep_comp(a, s.orb, comp, r1[0], r2[0], r3[0], r4[0], r5[0], r6[0], r7[0], r8[0]);
+            if (comp == 0) keep_x0h(a, s.orb, x0, h);
+        }
+    }
+    __device__ __forceinline__ void save_state(const WinArgs &w, int64_t N, const DopState &s,
+                                               const double (&y)[1], const double (&k1)[1],
+                                               int64_t flags) const {
+        if (writes) {
+            win_save_comp(w, N, s.orb, comp, y[0], k1[0]);
+            if (comp == 0) win_save_scalars(w, N, s, flags);
+        }
+    }
 };
 
 // one orbit per 4-lane quad, lane c < 3 holding components c and c + 3 (the
@@ -250,6 +398,7 @@ template <bool SCF, int LAY>
 struct QuadPair {
     static constexpr int NC = 2;
     int axis;
+    bool writes;  // lanes 0..2 of the quad store the dense samples
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[2], double (&f)[2]) const {
         const double qx = quad_bcast<0>(y[0]), qy = quad_bcast<1>(y[0]), qz = quad_bcast<2>(y[0]);
         double gx, gy, gz;
@@ -269,6 +418,38 @@ struct QuadPair {
         }
     }
     __device__ __forceinline__ double n() const { return 6.0; }
+    // dense sample s.oidx of orbit s.orb into w_out (6, out_rows, N)
+    __device__ __forceinline__ void store_dense(const DopArgs &a, const DopState &s,
+                                                const double (&v)[2]) const {
+        if (writes) {
+            const int64_t so = a.out_rows * a.N, at = (s.oidx - a.out_base) * a.N + s.orb;
+            a.w_out[axis * so + at] = v[0];
+            a.w_out[(axis + 3) * so + at] = v[1];
+        }
+    }
+    static constexpr bool WINDOWS = true;
+    __device__ __forceinline__ void keep_step(const DopArgs &a, const DopState &s,
+                                              const double (&r1)[2], const double (&r2)[2],
+                                              const double (&r3)[2], const double (&r4)[2],
+                                              const double (&r5)[2], const double (&r6)[2],
+                                              const double (&r7)[2], const double (&r8)[2],
+                                              double x0, double h) const {
+        if (writes) {
+            keep_comp(a, s.orb, axis, r1[0], r2[0], r3[0], r4[0], r5[0], r6[0], r7[0], r8[0]);
+            keep_comp(a, s.orb, axis + 3, r1[1], r2[1], r3[1], r4[1], r5[1], r6[1], r7[1],
+                      r8[1]);
+            if (axis == 0) keep_x0h(a, s.orb, x0, h);
+        }
+    }
+    __device__ __forceinline__ void save_state(const WinArgs &w, int64_t N, const DopState &s,
+                                               const double (&y)[2], const double (&k1)[2],
+                                               int64_t flags) const {
+        if (writes) {
+            win_save_comp(w, N, s.orb, axis, y[0], k1[0]);
+            win_save_comp(w, N, s.orb, axis + 3, y[1], k1[1]);
+            if (axis == 0) win_save_scalars(w, N, s, flags);
+        }
+    }
 };
 
 // N-body systems (Fwrapper_direct_nbody, dop853.cpp:990-1006): lane b of a
@@ -763,20 +944,7 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                     // samples the rest of it from the same polynomial
                     if (a.pend && s.oidx == a.out_end && s.oidx < a.n_out &&
                         in_step(x0, x1, a.t_out[s.oidx])) {
-                        double *pp = a.pend + s.orb;
-                        const int64_t N = a.N;
-                        FORN {
-                            pp[(0 * NC + c) * N] = r1[c];
-                            pp[(1 * NC + c) * N] = r2[c];
-                            pp[(2 * NC + c) * N] = r3[c];
-                            pp[(3 * NC + c) * N] = r4[c];
-                            pp[(4 * NC + c) * N] = r5[c];
-                            pp[(5 * NC + c) * N] = r6[c];
-                            pp[(6 * NC + c) * N] = r7[c];
-                            pp[(7 * NC + c) * N] = r8[c];
-                        }
-                        pp[(8 * NC) * N] = x0;
-                        pp[(8 * NC + 1) * N] = h;
+                        sys.keep_step(a, s, r1, r2, r3, r4, r5, r6, r7, r8, x0, h);
                         s.pend = 1;
                     }
                 }
@@ -819,12 +987,36 @@ __device__ __forceinline__ V from_lane(V v, int src) {
     return __shfl(v, src);
 }
 
+// Dense output in windows, after an attempt that returned `code`: keep the
+// state of an orbit that finished with a kept step, or that reached the
+// window's end (it then leaves the launch: returns true).  Nothing in the
+// last window or without windows.
+template <bool DENSE, class Sys, int NC>
+__device__ __forceinline__ bool window_pause(const DopArgs &a, const WinArgs &w, const Sys &sys,
+                                             const DopState &s, const double (&y)[NC],
+                                             const double (&k1)[NC], int code) {
+    if constexpr (DENSE) {
+        if (!a.pend || a.out_end >= a.n_out) return false;
+        if (code != 0) {
+            // finished: later windows skip it, or sample its kept step
+            sys.save_state(w, a.N, s, y, k1, s.pend ? WF_PEND : 0);
+            return false;
+        }
+        if (s.oidx >= a.out_end) {
+            sys.save_state(w, a.N, s, y, k1, WF_RUN | (s.pend ? WF_PEND : 0));
+            return true;
+        }
+    }
+    return false;
+}
+
 // The wide tail of one wave: the (at most WIDE_ORBITS) orbits still active
-// move to one 16-lane row each and are integrated to the end.  Called with
-// every lane of the wave converged; the queue is drained.
-template <bool SCF, int LAY>
-__device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, bool active,
-                                       const DopState &s6, const double (&y6)[6],
+// move to one 16-lane row each and are integrated to the end (or, in a
+// window of the dense output, to the window's end).  Called with every lane
+// of the wave converged; the queue is drained.
+template <bool SCF, bool DENSE, int LAY>
+__device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, const WinArgs &w,
+                                       bool active, const DopState &s6, const double (&y6)[6],
                                        const double (&k16)[6]) {
     const int lane = threadIdx.x & 63;
     const int row = lane / WIDE_ROW, c = lane % WIDE_ROW;
@@ -860,16 +1052,18 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
     s.reject = from_lane(s6.reject, src);
     s.iasti = from_lane(s6.iasti, src);
     s.nonsti = from_lane(s6.nonsti, src);
-    const WideRow<SCF, LAY> row_sys{comp};
+    s.pend = 0;
+    const WideRow<SCF, LAY> row_sys{comp, c < 6};
     // every row ends: each attempt advances nstep, and nmax bounds it
     while (__any(ract)) {
         if (ract) {
-            const int code = dop_attempt<SCF, false, LAY>(P, a, row_sys, s, y, k1);
+            const int code = dop_attempt<SCF, DENSE, LAY>(P, a, row_sys, s, y, k1);
             if (code != 0) {
                 if (c < 6) a.w_final[c * a.N + s.orb] = y[0];
                 if (c == 0) finish_stats(a, s, code);
                 ract = false;
             }
+            if (window_pause<DENSE>(a, w, row_sys, s, y, k1, code)) ract = false;
         }
     }
 }
@@ -878,9 +1072,9 @@ __device__ __forceinline__ void wide_tail(const DevPot &P, const DopArgs &a, boo
 // move to one 4-lane quad each; once at most WIDE_ORBITS remain they move on
 // to 16-lane rows (wide_tail).  Called with every lane of the wave converged;
 // the queue is drained.
-template <bool SCF, int LAY>
-__device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, bool active,
-                                       const DopState &s6, const double (&y6)[6],
+template <bool SCF, bool DENSE, int LAY>
+__device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, const WinArgs &w,
+                                       bool active, const DopState &s6, const double (&y6)[6],
                                        const double (&k16)[6]) {
     const int lane = threadIdx.x & 63;
     const int quad = lane >> 2, c = lane & 3;
@@ -920,11 +1114,12 @@ __device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, boo
     s.reject = from_lane(s6.reject, src);
     s.iasti = from_lane(s6.iasti, src);
     s.nonsti = from_lane(s6.nonsti, src);
-    const QuadPair<SCF, LAY> qsys{axis};
+    s.pend = 0;
+    const QuadPair<SCF, LAY> qsys{axis, c < 3};
     // every quad ends: each attempt advances nstep, and nmax bounds it
     while (__popcll(__ballot(qact && c == 0)) > WIDE_ORBITS) {
         if (qact) {
-            const int code = dop_attempt<SCF, false, LAY>(P, a, qsys, s, y, k1);
+            const int code = dop_attempt<SCF, DENSE, LAY>(P, a, qsys, s, y, k1);
             if (code != 0) {
                 if (c < 3) {
                     a.w_final[axis * a.N + s.orb] = y[0];
@@ -933,6 +1128,7 @@ __device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, boo
                 if (c == 0) finish_stats(a, s, code);
                 qact = false;
             }
+            if (window_pause<DENSE>(a, w, qsys, s, y, k1, code)) qact = false;
         }
     }
     // the remaining orbits back in one lane each (lane 0 of their quad), then rows
@@ -943,7 +1139,7 @@ __device__ __forceinline__ void quad_tail(const DevPot &P, const DopArgs &a, boo
         yb[j] = from_lane(j < 3 ? y[0] : y[1], from);
         kb[j] = from_lane(j < 3 ? k1[0] : k1[1], from);
     }
-    wide_tail<SCF, LAY>(P, a, qact && c == 0, s, yb, kb);
+    wide_tail<SCF, DENSE, LAY>(P, a, w, qact && c == 0, s, yb, kb);
 }
 
 // start the orbit at queue slot `slot`: state, first derivative and, for
@@ -974,169 +1170,6 @@ __device__ __forceinline__ bool load_orbit(const DevPot &P, const DopArgs &a, in
     return true;
 }
 
-template <bool SCF, bool DENSE, int LAY>
-__global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a) {
-    const DevPot &P = *Pp;
-    const int lane = threadIdx.x & 63;
-
-    bool active = false, exhausted = false;
-    DopState s;
-    s.orb = 0;
-    s.x = s.xend = s.h = s.hmax = 0.0;
-    s.posneg = 1.0;
-    s.facold = 1e-4;
-    s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.stiff = s.nrejct = s.nfcn = s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = s.pend = 0;
-    double y[6], k1[6];
-    FOR6 {
-        y[c] = 0.0;
-        k1[c] = 0.0;
-    }
-
-    if (!DENSE && a.n_tail > 0) {
-        // the first waves of the grid take the queue order's first n_tail
-        // (longest) orbits, four per wave, and integrate them in 16-lane rows
-        // from the start; the other waves share the rest of the queue
-        const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        if (wave * WIDE_ORBITS < a.n_tail) {
-            const int64_t slot = wave * WIDE_ORBITS + lane;
-            if (lane < WIDE_ORBITS && slot < a.n_tail)
-                active = load_orbit<SCF, LAY>(P, a, slot, s, y, k1);
-            wide_tail<SCF, LAY>(P, a, active, s, y, k1);
-            return;
-        }
-    }
-
-    while (true) {
-        // ---- refill idle lanes from the queue (one atomic per wave) ----
-        const bool need = !active && !exhausted;
-        const unsigned long long want = __ballot(need);
-        if (want) {
-            const int leader = __ffsll((long long)want) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(want));
-            base = __shfl(base, leader);
-            if (need) {
-                const unsigned long long below = want & ((1ull << lane) - 1ull);
-                const unsigned long long slot = base + (unsigned long long)__popcll(below);
-                const unsigned long long qslot = slot + (unsigned long long)a.n_tail;
-                if (qslot < (unsigned long long)a.N) {
-                    active = load_orbit<SCF, LAY>(P, a, (int64_t)qslot, s, y, k1);
-                } else {
-                    exhausted = true;
-                }
-            }
-        }
-        if (!__any(active)) break;
-        if (!DENSE && a.wide) {
-            // every idle lane has found the queue empty; with few orbits
-            // left, spread them over rows and finish them there
-            const int left = __popcll(__ballot(active));
-            if (left <= WIDE_ORBITS) {
-                wide_tail<SCF, LAY>(P, a, active, s, y, k1);
-                break;
-            }
-            if (left <= QUAD_ORBITS && a.wide > 1) {
-                quad_tail<SCF, LAY>(P, a, active, s, y, k1);
-                break;
-            }
-        }
-        if (!active) continue;
-
-        const int code = dop_attempt<SCF, DENSE, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
-        if (code != 0) {
-            FOR6 a.w_final[c * a.N + s.orb] = y[c];
-            finish_stats(a, s, code);
-            active = false;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Dense output in windows of the t_out grid (gamd_dop853_dense_stream).  The
-// launch for window [out_base, out_end) takes every orbit from where the
-// previous window left it, samples it into the window's slot and stops it at
-// the first step boundary past the window's last sample; its state is saved
-// and the slot is copied to host memory while the next window runs.  A step
-// that also covers the next window's first sample leaves its polynomial
-// (r1..r8, x0, h) behind, and the next window samples the rest of it before
-// the orbit steps on.  Steps, step sizes and samples are those of one
-// gamd_dop853 launch with w_out.
-//
-// Saved state, one row of N words per field:
-enum : int {
-    WS_X, WS_XEND, WS_H, WS_HMAX, WS_POSNEG, WS_FACOLD, WS_HLAMB,  // doubles
-    WS_NSTEP, WS_NACCPT, WS_NREJCT, WS_NFCN, WS_OIDX, WS_STIFF,    // int64
-    WS_REJECT, WS_IASTI, WS_NONSTI, WS_FLAGS,                       // int64
-    WS_Y, WS_K1 = WS_Y + 6, WS_WORDS = WS_K1 + 6
-};
-constexpr int WS_PEND_WORDS = 8 * 6 + 2;    // the kept step: r1..r8, x0, h
-constexpr int64_t WF_RUN = 1, WF_PEND = 2;  // still stepping; a kept step to sample
-
-struct WinArgs {
-    double *st;  // (WS_WORDS, N)
-    int first;   // the first window: orbits start from w0
-};
-
-__device__ __forceinline__ void win_save(const WinArgs &w, int64_t N, const DopState &s,
-                                         const double (&y)[6], const double (&k1)[6],
-                                         int64_t flags) {
-    double *d = w.st + s.orb;
-    int64_t *q = reinterpret_cast<int64_t *>(w.st) + s.orb;
-    d[WS_X * N] = s.x;
-    d[WS_XEND * N] = s.xend;
-    d[WS_H * N] = s.h;
-    d[WS_HMAX * N] = s.hmax;
-    d[WS_POSNEG * N] = s.posneg;
-    d[WS_FACOLD * N] = s.facold;
-    d[WS_HLAMB * N] = s.hlamb;
-    q[WS_NSTEP * N] = s.nstep;
-    q[WS_NACCPT * N] = s.naccpt;
-    q[WS_NREJCT * N] = s.nrejct;
-    q[WS_NFCN * N] = s.nfcn;
-    q[WS_OIDX * N] = s.oidx;
-    q[WS_STIFF * N] = s.stiff;
-    q[WS_REJECT * N] = s.reject;
-    q[WS_IASTI * N] = s.iasti;
-    q[WS_NONSTI * N] = s.nonsti;
-    q[WS_FLAGS * N] = flags;
-    FOR6 {
-        d[(WS_Y + c) * N] = y[c];
-        d[(WS_K1 + c) * N] = k1[c];
-    }
-}
-
-// the saved state of orbit orb; returns its flags
-__device__ __forceinline__ int64_t win_load(const WinArgs &w, int64_t N, int64_t orb,
-                                            DopState &s, double (&y)[6], double (&k1)[6]) {
-    const double *d = w.st + orb;
-    const int64_t *q = reinterpret_cast<const int64_t *>(w.st) + orb;
-    s.orb = orb;
-    s.x = d[WS_X * N];
-    s.xend = d[WS_XEND * N];
-    s.h = d[WS_H * N];
-    s.hmax = d[WS_HMAX * N];
-    s.posneg = d[WS_POSNEG * N];
-    s.facold = d[WS_FACOLD * N];
-    s.hlamb = d[WS_HLAMB * N];
-    s.nstep = q[WS_NSTEP * N];
-    s.naccpt = q[WS_NACCPT * N];
-    s.nrejct = q[WS_NREJCT * N];
-    s.nfcn = q[WS_NFCN * N];
-    s.oidx = q[WS_OIDX * N];
-    s.stiff = q[WS_STIFF * N];
-    s.reject = (int)q[WS_REJECT * N];
-    s.iasti = (int)q[WS_IASTI * N];
-    s.nonsti = (int)q[WS_NONSTI * N];
-    s.pend = 0;
-    FOR6 {
-        y[c] = d[(WS_Y + c) * N];
-        k1[c] = d[(WS_K1 + c) * N];
-    }
-    return q[WS_FLAGS * N];
-}
-
 // samples [s.oidx, out_end) of the step kept by the previous window; returns
 // true when the step still covers the sample after this window's last
 template <bool SCF, int LAY>
@@ -1160,13 +1193,37 @@ __device__ __forceinline__ bool sample_kept(const DopArgs &a, DopState &s) {
     return s.oidx < a.n_out && in_step(x0, x1, a.t_out[s.oidx]);
 }
 
-// one window: the persistent one-lane queue of k_dop853 over every orbit
-template <bool SCF, int LAY>
-__global__ __launch_bounds__(256) void k_dop853_window(const DevPot *__restrict__ Pp, DopArgs a,
-                                                       WinArgs w) {
+// the orbit at queue slot `slot` into one lane: from w0 (load_orbit), or, in
+// a later window of the dense output, from its kept state, after sampling
+// what its kept step covers of this window.  Returns whether it steps on in
+// this launch.
+template <bool SCF, bool DENSE, int LAY>
+__device__ __forceinline__ bool take_orbit(const DevPot &P, const DopArgs &a, const WinArgs &w,
+                                           int64_t slot, DopState &s, double (&y)[6],
+                                           double (&k1)[6]) {
+    if (!DENSE || !a.pend || w.first) return load_orbit<SCF, LAY>(P, a, slot, s, y, k1);
+    const int64_t orb = a.order ? (int64_t)a.order[slot] : slot;
+    // slots that name no orbit were skipped by the first window too
+    int64_t flags = 0;
+    if (orb >= 0 && orb < a.N) flags = win_load(w, a.N, orb, s, y, k1);
+    // an orbit behind this window (one a window did not visit) is not run:
+    // its samples would fall before the window's slot
+    if (s.oidx < a.out_base) flags = 0;
+    if (flags & WF_PEND) {
+        const bool more = sample_kept<SCF, LAY>(a, s);
+        flags = (flags & WF_RUN) | (more ? WF_PEND : 0);
+        if (more || !(flags & WF_RUN)) OneLane<SCF, LAY>{}.save_state(w, a.N, s, y, k1, flags);
+        if (more) flags = 0;  // no steps before its samples are out
+    }
+    return (flags & WF_RUN) != 0;
+}
+
+template <bool SCF, bool DENSE, int LAY>
+__global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, DopArgs a,
+                                                WinArgs w) {
     const DevPot &P = *Pp;
     const int lane = threadIdx.x & 63;
-    const bool last_window = a.out_end >= a.n_out;
+
     bool active = false, exhausted = false;
     DopState s;
     s.orb = 0;
@@ -1181,7 +1238,23 @@ __global__ __launch_bounds__(256) void k_dop853_window(const DevPot *__restrict_
         y[c] = 0.0;
         k1[c] = 0.0;
     }
+
+    if (a.n_tail > 0) {
+        // the first waves of the grid take the queue order's first n_tail
+        // (longest) orbits, four per wave, and integrate them in 16-lane rows
+        // from the start; the other waves share the rest of the queue
+        const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        if (wave * WIDE_ORBITS < a.n_tail) {
+            const int64_t slot = wave * WIDE_ORBITS + lane;
+            if (lane < WIDE_ORBITS && slot < a.n_tail)
+                active = take_orbit<SCF, DENSE, LAY>(P, a, w, slot, s, y, k1);
+            wide_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+            return;
+        }
+    }
+
     while (true) {
+        // ---- refill idle lanes from the queue (one atomic per wave) ----
         const bool need = !active && !exhausted;
         const unsigned long long want = __ballot(need);
         if (want) {
@@ -1192,40 +1265,45 @@ __global__ __launch_bounds__(256) void k_dop853_window(const DevPot *__restrict_
             if (need) {
                 const unsigned long long below = want & ((1ull << lane) - 1ull);
                 const unsigned long long slot = base + (unsigned long long)__popcll(below);
-                if (slot < (unsigned long long)a.N) {
-                    if (w.first) {
-                        active = load_orbit<SCF, LAY>(P, a, (int64_t)slot, s, y, k1);
-                    } else {
-                        const int64_t orb = a.order ? (int64_t)a.order[slot] : (int64_t)slot;
-                        // slots that name no orbit were skipped by the first window too
-                        int64_t flags = 0;
-                        if (orb >= 0 && orb < a.N) flags = win_load(w, a.N, orb, s, y, k1);
-                        if (flags & WF_PEND) {
-                            const bool more = sample_kept<SCF, LAY>(a, s);
-                            flags = (flags & WF_RUN) | (more ? WF_PEND : 0);
-                            if (more || !(flags & WF_RUN)) win_save(w, a.N, s, y, k1, flags);
-                            if (more) flags = 0;  // no steps before its samples are out
-                        }
-                        active = (flags & WF_RUN) != 0;
-                    }
+                const unsigned long long qslot = slot + (unsigned long long)a.n_tail;
+                if (qslot < (unsigned long long)a.N) {
+                    active = take_orbit<SCF, DENSE, LAY>(P, a, w, (int64_t)qslot, s, y, k1);
                 } else {
                     exhausted = true;
                 }
             }
         }
-        if (!__any(active)) break;
+        // Idle lanes that have not yet found the queue empty take slots again
+        // before the wave leaves or spreads its last orbits: a slot can name
+        // an orbit with nothing to do (in a window of the dense output, one
+        // that has finished), and every slot's orbit must be visited.
+        const bool searching = __any(!active && !exhausted);
+        if (!__any(active)) {
+            if (!searching) break;
+            continue;
+        }
+        if (a.wide && !searching) {
+            // every idle lane has found the queue empty; with few orbits
+            // left, spread them over rows and finish them there
+            const int left = __popcll(__ballot(active));
+            if (left <= WIDE_ORBITS) {
+                wide_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                break;
+            }
+            if (left <= QUAD_ORBITS && a.wide > 1) {
+                quad_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                break;
+            }
+        }
         if (!active) continue;
-        const int code = dop_attempt<SCF, true, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
+
+        const int code = dop_attempt<SCF, DENSE, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
         if (code != 0) {
             FOR6 a.w_final[c * a.N + s.orb] = y[c];
             finish_stats(a, s, code);
-            if (!last_window) win_save(w, a.N, s, y, k1, s.pend ? WF_PEND : 0);
-            active = false;
-        } else if (!last_window && s.oidx >= a.out_end) {
-            // every sample before the window's end is out: stop at this step
-            win_save(w, a.N, s, y, k1, WF_RUN | (s.pend ? WF_PEND : 0));
             active = false;
         }
+        if (window_pause<DENSE>(a, w, OneLane<SCF, LAY>{}, s, y, k1, code)) active = false;
     }
 }
 
@@ -1483,24 +1561,38 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 template <bool SCF, bool DENSE, int LAY>
-int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
-    auto kern = k_dop853<SCF, DENSE, LAY>;
+int64_t dop_grid(int block, int64_t N) {
+    // the persistent grid: every workgroup that fits at once, at most one
+    // lane per orbit (queried before any stream capture begins)
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853<SCF, DENSE, LAY>, block,
+                                                     0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
     int64_t blocks = (int64_t)per_cu * cu_count();
-    const int64_t needed = (a.N + block - 1) / block;
+    const int64_t needed = (N + block - 1) / block;
     if (blocks > needed) blocks = needed;
-    if (blocks < 1) blocks = 1;
-    DopArgs b = a;
-    // dedicated wide waves: at most half the grid's waves, never all of it
+    return blocks < 1 ? 1 : blocks;
+}
+
+// dedicated wide waves: at most half the grid's waves, never all of it
+int64_t cap_tail(const DopArgs &a, int64_t blocks, int block) {
     const int64_t waves = blocks * (block / 64);
-    int64_t max_tail = WIDE_ORBITS * (waves / 2);
-    if (DENSE || !a.wide) max_tail = 0;
-    if (b.n_tail > max_tail) b.n_tail = max_tail;
-    if (b.n_tail > a.N) b.n_tail = a.N;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, s, P, b);
+    const int64_t max_tail = a.wide ? WIDE_ORBITS * (waves / 2) : 0;
+    const int64_t t = a.n_tail < max_tail ? a.n_tail : max_tail;
+    return t < a.N ? t : a.N;
+}
+
+template <bool SCF, bool DENSE, int LAY>
+int launch(const DevPot *P, const DopArgs &a, int block, hipStream_t s) {
+    const int64_t blocks = dop_grid<SCF, DENSE, LAY>(block, a.N);
+    DopArgs b = a;
+    b.n_tail = cap_tail(a, blocks, block);
+    WinArgs w;
+    w.st = nullptr;
+    w.first = 1;
+    hipLaunchKernelGGL((k_dop853<SCF, DENSE, LAY>), dim3((unsigned)blocks), dim3(block), 0, s, P,
+                       b, w);
     return launch_status();
 }
 
@@ -1677,27 +1769,12 @@ extern "C" int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, 
 
 namespace {
 
-// the window kernel's persistent grid: every workgroup that fits at once,
-// at most one lane per orbit (queried before any capture begins)
-template <bool SCF, int LAY>
-int64_t window_grid(int block, int64_t N) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dop853_window<SCF, LAY>, block,
-                                                     0) != hipSuccess ||
-        per_cu <= 0)
-        per_cu = 1;
-    int64_t blocks = (int64_t)per_cu * cu_count();
-    const int64_t needed = (N + block - 1) / block;
-    if (blocks > needed) blocks = needed;
-    return blocks < 1 ? 1 : blocks;
-}
-
 // the window launches and their copies; kernels on s, copies on c
 template <bool SCF, int LAY>
 int dense_windows(const DevPot *P, DopArgs a, int block, int64_t blocks, int64_t W,
                   double *host_out, double *ring, WinArgs w, hipStream_t s, hipStream_t c,
                   hipEvent_t comp, hipEvent_t copied[2]) {
-    auto kern = k_dop853_window<SCF, LAY>;
+    auto kern = k_dop853<SCF, true, LAY>;
     const int64_t n_out = a.n_out, N = a.N;
     const int64_t nwin = (n_out + W - 1) / W;
     if (a.order && a.status) {
@@ -1754,9 +1831,10 @@ extern "C" int gamd_dop853_dense_stream(const gamd_potential *pot, const double 
                                         double hmax, const double *t_out, size_t n_out,
                                         int64_t window_rows, int use_graph, double *host_out,
                                         double *w_final, int32_t *status, int64_t *stats,
-                                        const int32_t *order, int block, void *workspace,
-                                        size_t workspace_bytes, void *stream) {
-    int rc = dop853_check(pot, w0, t0, t1, w_final, host_out, t_out, n_out, 0, block, nmax,
+                                        const int32_t *order, int64_t n_tail, int block,
+                                        void *workspace, size_t workspace_bytes,
+                                        void *stream) {
+    int rc = dop853_check(pot, w0, t0, t1, w_final, host_out, t_out, n_out, n_tail, block, nmax,
                           nstiff);
     if (rc != GAMD_OK) return rc;
     if (!host_out || !t_out || n_out == 0 || window_rows < 1 || !workspace)
@@ -1769,8 +1847,7 @@ extern "C" int gamd_dop853_dense_stream(const gamd_potential *pot, const double 
     double *ring = reinterpret_cast<double *>(static_cast<char *>(workspace) + 256);
     double *st = ring + 2 * 6 * (size_t)window_rows * N;
     DopArgs a = dop_args(w0, N, t0, t1, h0, rtol, atol, nmax, nstiff, hmax, t_out, n_out, nullptr,
-                         w_final, status, stats, order, 0, queue);
-    a.wide = 0;
+                         w_final, status, stats, order, n_tail, queue);
     a.pend = st + (size_t)WS_WORDS * N;
     WinArgs w;
     w.st = st;
@@ -1785,9 +1862,10 @@ extern "C" int gamd_dop853_dense_stream(const gamd_potential *pot, const double 
     auto run = scf  ? dense_windows<true, LAYOUT_GENERIC>
                : mw ? dense_windows<false, LAYOUT_MW>
                     : dense_windows<false, LAYOUT_GENERIC>;
-    const int64_t blocks = scf  ? window_grid<true, LAYOUT_GENERIC>(block, (int64_t)N)
-                           : mw ? window_grid<false, LAYOUT_MW>(block, (int64_t)N)
-                                : window_grid<false, LAYOUT_GENERIC>(block, (int64_t)N);
+    const int64_t blocks = scf  ? dop_grid<true, true, LAYOUT_GENERIC>(block, (int64_t)N)
+                           : mw ? dop_grid<false, true, LAYOUT_MW>(block, (int64_t)N)
+                                : dop_grid<false, true, LAYOUT_GENERIC>(block, (int64_t)N);
+    a.n_tail = cap_tail(a, blocks, block);
     rc = GAMD_ERR_HIP;
     if (hipStreamCreateWithFlags(&c, hipStreamNonBlocking) != hipSuccess) goto done;
     if (hipEventCreateWithFlags(&comp, hipEventDisableTiming) != hipSuccess ||

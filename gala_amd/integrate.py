@@ -182,6 +182,8 @@ def ruth4(potential, w0, dt, n_steps, save_stride=1, save=True, t0=0.0):
 # launch 114.3-114.6 ms either way) in half the pilot time (3.2 against 5.7-6.1 ms);
 # at 0.5 % the ranking starts to cost the full launch
 PILOT_FRACTION = 0.01
+# integrate_orbit orders per-orbit DOP853 runs of at least this many orbits by a pilot pass
+PILOT_MIN_ORBITS = 4096
 
 
 def pilot_order(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstiff=0,
@@ -246,14 +248,14 @@ def dop853(potential, w0, t0, t1, h0=None, rtol=1e-10, atol=1e-10, nmax=0, nstif
 
 
 def dop853_to_host(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e-10, nmax=0,
-                   nstiff=0, hmax=0.0, order=None, block=0, window_rows=None, graph=None,
-                   out=None):
+                   nstiff=0, hmax=0.0, order=None, n_tail=0, block=0, window_rows=None,
+                   graph=None, out=None):
     """dop853(..., t_out=t_out) with the dense output streamed to pinned host
     memory while the integration runs (gamd_dop853_dense_stream): the t_out
     grid is taken in windows of window_rows samples, each window's samples
     copied on a second stream while the next window integrates.  window_rows
     / graph: None = the autotuner's remembered choice for this problem, else
-    a default of ~16 windows.  order: a permutation or "pilot" (as dop853).
+    a default of ~16 windows.  order, n_tail, block: as dop853 (speed knobs).
     out: a pinned host float64 tensor (6, n_out, N) to reuse.  Returns
     dict(w_out = numpy view of the pinned buffer (6, n_out, N), w_final,
     status, stats (device)), equal bit for bit to dop853's."""
@@ -298,7 +300,7 @@ def dop853_to_host(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e-10
                                             float(hmax), tout.data_ptr(), n_out, window_rows,
                                             int(bool(graph)), out.data_ptr(), wfin.data_ptr(),
                                             status.data_ptr(), stats.data_ptr(), dev.ptr(ordd),
-                                            int(block), ws.data_ptr(), nbytes,
+                                            int(n_tail), int(block), ws.data_ptr(), nbytes,
                                             dev.stream_ptr()),
                "gamd_dop853_dense_stream")
     return {"w_out": out.numpy(), "w_final": wfin, "status": status, "stats": stats}
@@ -476,6 +478,12 @@ def integrate_hamiltonian(hamiltonian, w0, Integrator=None, Integrator_kwargs=No
                   hmax=Integrator_kwargs.get("dt_max", 0.0),
                   t_out=t if save_all else None)
         nbatch = int(Integrator_kwargs.get("nbatch", 100))  # chamiltonian.pyx:348
+        if nbatch == 1:
+            # many orbits: longest first from a pilot pass, the longest on
+            # 16-lane rows (the same results for any order and n_tail)
+            N = arr.shape[1]
+            if N >= PILOT_MIN_ORBITS:
+                kw.update(order="pilot", n_tail=min(1024, N // 64))
         if nbatch == 1 and save_all:
             # the dense output streams to pinned host memory while it is computed
             kw.pop("t_out")

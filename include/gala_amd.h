@@ -171,10 +171,11 @@ int gamd_ruth4(const gamd_potential *pot, const double *w0, size_t N, double t0,
  * GAMD_STATUS_NOT_RUN for every orbit; entries outside 0..N-1 are skipped,
  * so an orbit that the order does not name keeps that status (the device
  * array is not read back to check it; gala_amd.integrate.dop853 checks that
- * a caller's order is a permutation before the call).  n_tail >= 0: the first n_tail orbits of that order are
- * integrated from the start by dedicated waves, four orbits per wave, each
- * spread over 16 lanes (capped at half the grid; ignored with dense output);
- * results are identical for every n_tail.  block: threads per workgroup,
+ * a caller's order is a permutation before the call).  n_tail >= 0: the
+ * first n_tail orbits of that order are integrated from the start by
+ * dedicated waves, four orbits per wave, each spread over 16 lanes (capped
+ * at half the grid); results, dense output included, are identical for
+ * every n_tail.  block: threads per workgroup,
  * 64, 128, 192 or 256 (0 = 256); results are identical for every block.
  * Both are speed knobs only: gala_amd.autotune times them on the caller's
  * workload. */
@@ -216,8 +217,9 @@ int gamd_dop853_noalloc(const gamd_potential *pot, const double *w0, size_t N,
  * (HOST, ideally pinned, (6, n_out, N)) on a second stream while the next
  * window runs.  Steps, samples, w_final, status and stats equal gamd_dop853's
  * with w_out bit for bit; a sample outside an orbit's [t0, t1] is not
- * written (gamd_dop853 leaves it unwritten too).  Orbits run one per lane
- * (no n_tail rows, as gamd_dop853 with dense output).  use_graph != 0
+ * written (gamd_dop853 leaves it unwritten too).  The launch layouts are
+ * gamd_dop853's (order, n_tail dedicated rows, quads and rows at the queue's
+ * end; a row or quad stops at a window's end as a lane does).  use_graph != 0
  * captures the window kernels and copies as one HIP graph and launches that
  * (no memset nodes).  workspace: device,
  * gamd_dop853_dense_workspace_bytes(N, window_rows) bytes (the work-queue
@@ -231,8 +233,8 @@ int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t
                              double atol, int64_t nmax, int64_t nstiff, double hmax,
                              const double *t_out, size_t n_out, int64_t window_rows,
                              int use_graph, double *host_out, double *w_final, int32_t *status,
-                             int64_t *stats, const int32_t *order, int block, void *workspace,
-                             size_t workspace_bytes, void *stream);
+                             int64_t *stats, const int32_t *order, int64_t n_tail, int block,
+                             void *workspace, size_t workspace_bytes, void *stream);
 
 #define GAMD_DOP853_MAX_NBATCH 1024  /* orbits of one batched system (one workgroup) */
 

@@ -196,3 +196,41 @@ def test_dedicated_tail_waves_match():
             r = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, order=order, n_tail=n_tail)
             for k, a in zip(("w_final", "status", "stats"), ref):
                 assert np.array_equal(dev.to_host(r[k]), a), (k, n_tail)
+
+
+@pytest.mark.parametrize("layout", ["milky_way", "generic"])
+def test_dense_output_same_in_every_lane_layout(layout, monkeypatch):
+    """With dense output too, the orbits that set a launch's length move to
+    4-lane quads and 16-lane rows (at the queue's end, and from the start
+    for the n_tail longest); every sample, final state, status and step count
+    equals one lane per orbit to the end, bit for bit."""
+    from gala_amd.integrate import dop853, pilot_order
+    from gala_amd import _device as dev
+    from gala_amd.potential import HernquistPotential, PlummerPotential
+    orc = mw_oracle()
+    w0, T = workloads.composite_dop853(vcirc_oracle(orc), n=3000, seed=27)
+    T = T * 0.05
+    t_out = np.linspace(0.0, 500.0, 33)
+    if layout == "milky_way":
+        pot = mw_product()
+    else:
+        pot = HernquistPotential(m=1e11, c=2.0) + PlummerPotential(m=1e10, b=1.0,
+                                                                   origin=[1.0, 0.5, -0.2])
+    keys = ("w_out", "w_final", "status", "stats")
+    monkeypatch.setenv("GAMD_DOP853_WIDE", "0")
+    ref = [dev.to_host(v) for v in
+           (dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, t_out=t_out)[k] for k in keys)]
+    st = ref[2]
+    reached = (t_out[:, None] <= T[None, :]) & (st == 1)[None, :]
+    print(f"{reached.sum()} samples reached of {reached.size}")
+    order = pilot_order(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10)
+    for wide, n_tail, o in (("1", 0, None), ("2", 0, None), ("2", 256, order),
+                            ("2", 100000, order)):
+        monkeypatch.setenv("GAMD_DOP853_WIDE", wide)
+        r = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, t_out=t_out, order=o,
+                   n_tail=n_tail)
+        got = [dev.to_host(r[k]) for k in keys]
+        m = np.broadcast_to(reached[None], ref[0].shape)
+        assert np.array_equal(got[0][m], ref[0][m]), (wide, n_tail)
+        for k, a, b in zip(keys[1:], ref[1:], got[1:]):
+            assert np.array_equal(a, b), (wide, n_tail, k)

@@ -43,7 +43,9 @@ def _check_equal(direct, streamed, mask, label):
 def test_windows_match_one_launch():
     """3,000 orbits of the uneven workload (0.1-10 Gyr each) sampled at 41
     shared times: orbits finish in different windows and, with one sample per
-    window, many steps span several windows."""
+    window, many steps span several windows.  Every lane layout stops at a
+    window's end: one lane per orbit, and the 4-lane quads and 16-lane rows
+    that take a wave's last orbits and the n_tail longest."""
     from gala_amd.integrate import dop853, dop853_to_host, pilot_order
     pot = mw_product()
     w0, T = workloads.composite_dop853(vcirc_oracle(mw_oracle()), n=3000, seed=31)
@@ -62,6 +64,31 @@ def test_windows_match_one_launch():
     r = dop853_to_host(pot, w0, 0.0, T, t_out, h0=1.0, rtol=1e-8, atol=1e-10, order=order,
                        window_rows=3, graph=True)
     _check_equal(direct, r, mask, "pilot order")
+    # the longest orbits on dedicated 16-lane rows, stopping at each window's end
+    for rows, n_tail in ((1, 64), (5, 512), (41, 512)):
+        r = dop853_to_host(pot, w0, 0.0, T, t_out, h0=1.0, rtol=1e-8, atol=1e-10, order=order,
+                           n_tail=n_tail, window_rows=rows, graph=False)
+        _check_equal(direct, r, mask, f"rows {rows} n_tail {n_tail}")
+
+
+def test_windows_visit_every_orbit_when_slots_outnumber_lanes():
+    """131,072 short orbits, more than the persistent grid has lanes, in a
+    shuffled queue order with one sample per window: in later windows most
+    queue slots name orbits that have finished, and a lane that draws one
+    must draw again until the queue is empty, so that no running orbit is
+    left out of a window."""
+    from gala_amd.integrate import dop853, dop853_to_host
+    pot = mw_product()
+    w0, T = workloads.composite_dop853(vcirc_oracle(mw_oracle()), n=131072, seed=33)
+    T = T * 0.02
+    t_out = np.linspace(0.0, 200.0, 21)
+    mask = _written(t_out, np.zeros_like(T), T)
+    order = np.random.default_rng(4).permutation(T.size).astype(np.int32)
+    direct = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, t_out=t_out)
+    for rows, n_tail in ((1, 0), (1, 1024), (4, 256)):
+        r = dop853_to_host(pot, w0, 0.0, T, t_out, h0=1.0, rtol=1e-8, atol=1e-10, order=order,
+                           n_tail=n_tail, window_rows=rows, graph=False)
+        _check_equal(direct, r, mask, f"rows {rows} n_tail {n_tail}")
 
 
 def test_windows_generic_scf_and_backward():
@@ -110,7 +137,7 @@ def test_argument_checks():
         return lib.gamd_dop853_dense_stream(pot.c_handle, w0.data_ptr(), 10, t0.data_ptr(),
                                             t1.data_ptr(), None, 1e-8, 1e-8, 0, 0, 0.0,
                                             tout.data_ptr(), 5, rows, 0, host, wf.data_ptr(),
-                                            None, None, None, 0, ws.data_ptr(), nbytes,
+                                            None, None, None, 0, 0, ws.data_ptr(), nbytes,
                                             dev.stream_ptr())
     invalid = -1  # GAMD_ERR_INVALID
     assert call(0, out.data_ptr(), need) == invalid
