@@ -233,6 +233,61 @@ def bench_dop853_batched(args, w0_all):
     return res
 
 
+def bench_dop853_dense_to_host(args, w0_all, T_all):
+    """DOP853 dense output delivered to host memory (what integrate_orbit
+    with nbatch=1 returns): the first 262,144 orbits of the uneven workload
+    (their own times, up to 10 Gyr) sampled at 101 shared times over 10 Gyr,
+    in the pilot's longest-first order with 1,024 orbits on dedicated rows.
+    Direct writes (all samples in HBM, then one copy to pinned host memory)
+    against gamd_dop853_dense_stream (windows of the time grid, each copied on
+    a second stream while the next integrates; window size and graph capture
+    autotuned).  Wall clock to a complete host buffer."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import autotune
+    from gala_amd.integrate import dop853, dop853_to_host, pilot_order
+    from gala_amd.potential import MilkyWayLikePotential
+    pot = MilkyWayLikePotential()
+    n = min(args.n or 262144, w0_all.shape[1], 262144)
+    w0d = dev.to_dev(np.ascontiguousarray(w0_all[:, :n]))
+    t1d = dev.to_dev(np.ascontiguousarray(T_all[:n]))
+    t_out = np.linspace(0.0, 10000.0, 101)
+    kw = dict(rtol=1e-8, atol=1e-10)
+    order = pilot_order(pot, w0d, 0.0, t1d, 1.0, **kw)
+    host = torch.empty((6, t_out.size, n), dtype=torch.float64, pin_memory=True)
+
+    def direct():
+        r = dop853(pot, w0d, 0.0, t1d, 1.0, t_out=t_out, order=order, n_tail=1024, **kw)
+        host.copy_(r["w_out"])
+        torch.cuda.synchronize()
+
+    tuned = autotune.tune_dop853_dense(pot, w0d, 0.0, t1d, t_out, h0=1.0, order=order,
+                                       n_tail=1024, log=log, **kw)
+
+    def streamed():
+        dop853_to_host(pot, w0d, 0.0, t1d, t_out, h0=1.0, order=order, n_tail=1024, out=host,
+                       **kw)
+
+    direct()
+    t_d, t_s = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        direct()
+        t_d.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        streamed()
+        t_s.append(time.perf_counter() - t0)
+    gb = 48.0 * t_out.size * n / 1e9
+    log(f"dop853 dense output to host ({n} orbits x {t_out.size} samples, {gb:.2f} GB): "
+        f"direct {1e3 * min(t_d):.1f} ms, streamed {1e3 * min(t_s):.1f} ms "
+        f"({tuned['window_rows']} samples/window, graph {int(tuned['graph'])})")
+    del host
+    return {"n_orbits": n, "n_out": int(t_out.size), "GB": gb,
+            "direct_ms": 1e3 * min(t_d), "streamed_ms": 1e3 * min(t_s),
+            "window_rows": tuned["window_rows"], "graph": tuned["graph"],
+            "autotune_ms": tuned["trials_ms"]}
+
+
 def _energy(pot, w):
     """exact H = Phi + |v|^2/2 of device states w (6, N), on device."""
     from gala_amd import _device as dev
@@ -581,6 +636,8 @@ def main():
                     help="skip the composite leapfrog measurements")
     ap.add_argument("--lf-n", type=int, default=0, help="override the large leapfrog orbit count")
     ap.add_argument("--no-stream", action="store_true", help="skip the mock-stream measurements")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the DOP853 dense-output-to-host measurement")
     ap.add_argument("--scf-n", type=int, default=0, help="override the SCF orbit count")
     ap.add_argument("--scf-steps", type=int, default=0, help="override the SCF step count")
     ap.add_argument("--train-points", type=int, default=16 * 1024 * 1024)
@@ -600,6 +657,8 @@ def main():
     if not args.no_leapfrog:
         out["leapfrog_composite"] = bench_leapfrog(args)
     out["dop853_default_batching"] = bench_dop853_batched(args, w0)
+    if not args.no_dense:
+        out["dop853_dense_to_host"] = bench_dop853_dense_to_host(args, w0, T)
     if not args.no_stream:
         out["mockstream"] = bench_mockstream(args)
     if not args.no_scf:

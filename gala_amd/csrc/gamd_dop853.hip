@@ -1273,26 +1273,49 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
                 }
             }
         }
-        // Idle lanes that have not yet found the queue empty take slots again
-        // before the wave leaves or spreads its last orbits: a slot can name
-        // an orbit with nothing to do (in a window of the dense output, one
-        // that has finished), and every slot's orbit must be visited.
-        const bool searching = __any(!active && !exhausted);
-        if (!__any(active)) {
-            if (!searching) break;
-            continue;
-        }
-        if (a.wide && !searching) {
-            // every idle lane has found the queue empty; with few orbits
-            // left, spread them over rows and finish them there
-            const int left = __popcll(__ballot(active));
-            if (left <= WIDE_ORBITS) {
-                wide_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+        // A slot can name an orbit with nothing to do (one that has finished,
+        // in a window of the dense output, or an order entry that names no
+        // orbit): idle lanes that have not yet found the queue empty take
+        // slots again before the wave leaves or spreads its last orbits, so
+        // that every slot's orbit is visited.  Without dense output the test
+        // stays off the attempt loop's common path (a ballot per attempt cost
+        // ~10 % of the full launch); the dense kernel takes it every pass.
+        if constexpr (DENSE) {
+            const bool searching = __any(!active && !exhausted);
+            if (!__any(active)) {
+                if (!searching) break;
+                continue;
+            }
+            if (a.wide && !searching) {
+                // every idle lane has found the queue empty; with few orbits
+                // left, spread them over rows and finish them there
+                const int left = __popcll(__ballot(active));
+                if (left <= WIDE_ORBITS) {
+                    wide_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                    break;
+                }
+                if (left <= QUAD_ORBITS && a.wide > 1) {
+                    quad_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                    break;
+                }
+            }
+        } else {
+            if (!__any(active)) {
+                if (__any(!exhausted)) continue;
                 break;
             }
-            if (left <= QUAD_ORBITS && a.wide > 1) {
-                quad_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
-                break;
+            const int left = a.wide ? __popcll(__ballot(active)) : 64;
+            if (left <= QUAD_ORBITS && !__any(!active && !exhausted)) {
+                // every idle lane has found the queue empty; with few orbits
+                // left, spread them over rows and finish them there
+                if (left <= WIDE_ORBITS) {
+                    wide_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                    break;
+                }
+                if (a.wide > 1) {
+                    quad_tail<SCF, DENSE, LAY>(P, a, w, active, s, y, k1);
+                    break;
+                }
             }
         }
         if (!active) continue;
