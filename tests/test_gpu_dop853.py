@@ -234,3 +234,43 @@ def test_dense_output_same_in_every_lane_layout(layout, monkeypatch):
         assert np.array_equal(got[0][m], ref[0][m]), (wide, n_tail)
         for k, a, b in zip(keys[1:], ref[1:], got[1:]):
             assert np.array_equal(a, b), (wide, n_tail, k)
+
+
+def test_order_entries_naming_no_orbit_are_skipped():
+    """gamd_dop853 skips order entries outside 0..N-1 and keeps
+    GAMD_STATUS_NOT_RUN (0) for the orbits the order does not name.  Here
+    the first 200,000 of 262,144 slots name no orbit (more slots than the
+    persistent grid has lanes): waves whose draws bring nothing must draw
+    again, so every named orbit still runs, with the results of a run over
+    the named orbits alone."""
+    import torch
+    from gala_amd import _device as dev
+    from gala_amd import _lib
+    from gala_amd.integrate import dop853
+    lib = _lib.load()
+    pot = mw_product()
+    n = 262144
+    w0, T = workloads.composite_dop853(vcirc_oracle(mw_oracle()), n=n, seed=41)
+    T = T * 0.01
+    named = np.sort(np.random.default_rng(2).choice(n, n - 200000, replace=False))
+    order = np.concatenate([np.full(200000, -1), named]).astype(np.int32)
+    w0d, t1 = dev.to_dev(w0), dev.to_dev(T)
+    t0 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    h0 = torch.ones(n, dtype=torch.float64, device="cuda")
+    wfin = torch.full((6, n), np.nan, dtype=torch.float64, device="cuda")
+    status = torch.full((n,), 99, dtype=torch.int32, device="cuda")
+    stats = torch.zeros((n, 4), dtype=torch.int64, device="cuda")
+    ordd = dev.to_dev(order, dtype=torch.int32)
+    print(f"{n} slots, {named.size} naming orbits", flush=True)
+    rc = lib.gamd_dop853(pot.c_handle, w0d.data_ptr(), n, t0.data_ptr(), t1.data_ptr(),
+                         h0.data_ptr(), 1e-8, 1e-10, 0, 0, 0.0, None, 0, None, wfin.data_ptr(),
+                         status.data_ptr(), stats.data_ptr(), ordd.data_ptr(), 0, 0,
+                         dev.stream_ptr())
+    assert rc == 0
+    st = dev.to_host(status)
+    assert np.all(st[np.setdiff1d(np.arange(n), named)] == 0)
+    ref = dop853(pot, np.ascontiguousarray(w0[:, named]), 0.0, np.ascontiguousarray(T[named]),
+                 1.0, rtol=1e-8, atol=1e-10)
+    assert np.array_equal(st[named], dev.to_host(ref["status"]))
+    assert np.array_equal(dev.to_host(wfin)[:, named], dev.to_host(ref["w_final"]))
+    assert np.array_equal(dev.to_host(stats)[named], dev.to_host(ref["stats"]))
