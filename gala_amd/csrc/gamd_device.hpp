@@ -308,6 +308,28 @@ __host__ __device__ __forceinline__ int scf_pack_index(int l, int m, int n, int 
     return (nmax + 1) * (l * (l + 1) / 2) + n * (l + 1) + m;
 }
 
+// The Gegenbauer recurrence factors of scf_eval_t as a table over (l, n):
+// for a = 2l + 3/2 and b = a + 1, [0] = n + a - 1, [1] = n + 2a - 2,
+// [2] = n + b - 1, [3] = n + 2b - 2 (the same doubles the loop would form from
+// its counter), so the loop reads them instead of converting and adding.
+struct ScfRecTable {
+    double f[GAMD_SCF_MAX_LMAX + 1][GAMD_SCF_MAX_NMAX + 2][4];
+};
+constexpr ScfRecTable scf_rec_table() {
+    ScfRecTable t{};
+    for (int l = 0; l <= GAMD_SCF_MAX_LMAX; l++) {
+        const double a = 2.0 * l + 1.5, b = a + 1.0;
+        for (int n = 0; n <= GAMD_SCF_MAX_NMAX + 1; n++) {
+            t.f[l][n][0] = n + a - 1.0;
+            t.f[l][n][1] = n + 2.0 * a - 2.0;
+            t.f[l][n][2] = n + b - 1.0;
+            t.f[l][n][3] = n + 2.0 * b - 2.0;
+        }
+    }
+    return t;
+}
+static __constant__ ScfRecTable scf_rec = scf_rec_table();
+
 template <int NMAX, int LMAX>
 __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double G, double M,
                                            double rs, int nmax_rt, int lmax_rt, double x,
@@ -403,12 +425,12 @@ __device__ __forceinline__ void scf_eval_t(const double *__restrict__ cf, double
                 Cn = 2.0 * a * xi;
                 En = 1.0;
             } else {
-                Cn = (2.0 * xi * (n + a - 1.0) * c0 - (n + 2.0 * a - 2.0) * c1) * scf_inv_n[n];
+                const double *rn = scf_rec.f[l][n], *rk = scf_rec.f[l][n - 1];
+                Cn = (2.0 * xi * rn[0] * c0 - rn[1] * c1) * scf_inv_n[n];
                 const double b = a + 1.0;
                 const int k = n - 1;  // index of E
                 En = (k == 1) ? 2.0 * b * xi
-                              : (2.0 * xi * (k + b - 1.0) * e0 - (k + 2.0 * b - 2.0) * e1) *
-                                    scf_inv_n[k];
+                              : (2.0 * xi * rk[2] * e0 - rk[3] * e1) * scf_inv_n[k];
             }
             c1 = c0;
             c0 = Cn;
