@@ -493,7 +493,7 @@ def bench_mockstream(args):
             # bounds the kernel, not the FP64 pipe
             out[integ]["roofline"] = {
                 "bound": "latency", "pipe": "fp64 (VALU)",
-                "kernel": "k_mockstream (release-and-integrate)",
+                "kernel": "k_mockstream_pair (release-and-integrate, progenitor copy and particle on a lane pair)",
                 "achieved": flops / (kms / 1e3) / 1e12, "peak": roofline.PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": flops / (kms / 1e3) / 1e12 / roofline.PEAK_FP64_TFLOPS}
         log(f"mock stream ({integ}): {n} particles, {wall * 1e3:.1f} ms end to end")

@@ -361,6 +361,88 @@ __global__ __launch_bounds__(64) void k_mockstream(const DevPot *__restrict__ Ep
     out[5 * n + i] = p.vz;
 }
 
+// The one-body stream (the progenitor alone, gala's common case) on lane
+// pairs: lane 2i carries particle i's copy of the progenitor, lane 2i + 1
+// the particle.  Both lanes drift, evaluate the external gradient at their
+// own positions and kick with the same instructions, so the two external
+// evaluations of a step run side by side instead of one after the other;
+// the particle lane takes the progenitor's drifted position from its
+// partner and adds the progenitor's own potential.  Each lane's arithmetic
+// is that of bodies_step / particle_step (same expressions, same order), so
+// the results equal k_mockstream's bit for bit.
+template <bool SCF, int LAY>
+__global__ __launch_bounds__(64) void k_mockstream_pair(const DevPot *__restrict__ Ep, BodyTable bt,
+                                                        const double *__restrict__ traj,
+                                                        int64_t nfull, double t0, double dt,
+                                                        const double *__restrict__ w0,
+                                                        const double *__restrict__ rel_t,
+                                                        int64_t n, double tfinal,
+                                                        double *__restrict__ out) {
+    const DevPot &E = *Ep;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t i = g >> 1;             // the particle of this pair
+    const bool part = (g & 1) != 0;       // the particle's lane, else the progenitor's
+    const int64_t ii = i < n ? i : n - 1;  // lanes past the last pair mirror it (no stores)
+    int64_t idx = (int64_t)((rel_t[ii] - t0) / dt + 0.5);
+    if (idx < 0) idx = 0;
+    if (idx >= nfull) idx = nfull - 1;
+    const int64_t nsteps = (int64_t)((tfinal - rel_t[ii]) / dt + 0.5);
+    double X[3], V[3], H[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        X[c] = part ? w0[c * n + ii] : traj[(c * nfull + idx) * 1];
+        V[c] = part ? w0[(3 + c) * n + ii] : traj[((3 + c) * nfull + idx) * 1];
+    }
+    const bool body_pot = bt.p[0] != nullptr;
+    // c_init_velocity_nbody: the progenitor at its release state, the
+    // particle against the progenitor there
+    {
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(E, X[0], X[1], X[2], gx, gy, gz);
+        const double bx = __shfl_xor(X[0], 1), by = __shfl_xor(X[1], 1), bz = __shfl_xor(X[2], 1);
+        if (part && body_pot) {
+            double fx, fy, fz;
+            body_grad<SCF>(*bt.p[0], X[0], X[1], X[2], bx, by, bz, fx, fy, fz);
+            gx += fx;
+            gy += fy;
+            gz += fz;
+        }
+        H[0] = V[0] - gx * dt / 2.0;
+        H[1] = V[1] - gy * dt / 2.0;
+        H[2] = V[2] - gz * dt / 2.0;
+    }
+    for (int64_t s = 0; s < nsteps; s++) {
+        X[0] = X[0] + H[0] * dt;
+        X[1] = X[1] + H[1] * dt;
+        X[2] = X[2] + H[2] * dt;
+        double gx, gy, gz;
+        pot_grad_l<SCF, LAY>(E, X[0], X[1], X[2], gx, gy, gz);
+        // the progenitor's position after this step's drift
+        const double bx = __shfl_xor(X[0], 1), by = __shfl_xor(X[1], 1), bz = __shfl_xor(X[2], 1);
+        if (part && body_pot) {
+            double fx, fy, fz;
+            body_grad<SCF>(*bt.p[0], X[0], X[1], X[2], bx, by, bz, fx, fy, fz);
+            gx += fx;
+            gy += fy;
+            gz += fz;
+        }
+        V[0] = H[0] - gx * dt / 2.0;
+        V[1] = H[1] - gy * dt / 2.0;
+        V[2] = H[2] - gz * dt / 2.0;
+        H[0] = H[0] - gx * dt;
+        H[1] = H[1] - gy * dt;
+        H[2] = H[2] - gz * dt;
+    }
+    if (part && i < n) {
+        out[i] = X[0];
+        out[n + i] = X[1];
+        out[2 * n + i] = X[2];
+        out[3 * n + i] = V[0];
+        out[4 * n + i] = V[1];
+        out[5 * n + i] = V[2];
+    }
+}
+
 // The external potential's evaluator: the straight-line Milky-Way form when
 // the table has that layout and no body is an SCF expansion (as the DOP853
 // N-body kernels choose), else the general component loop, with the SCF
@@ -476,10 +558,20 @@ extern "C" int gamd_mockstream_leapfrog(const gamd_potential *ext,
         return GAMD_ERR_INVALID;
     if (n == 0) return GAMD_OK;
     const int ev = evaluator(ext, bt);
+    const unsigned block = 64;
+    if (nbody == 1) {
+        // the progenitor alone: lane pairs (k_mockstream_pair)
+        auto pair = ev == EV_SCF  ? k_mockstream_pair<true, LAYOUT_GENERIC>
+                    : ev == EV_MW ? k_mockstream_pair<false, LAYOUT_MW>
+                                  : k_mockstream_pair<false, LAYOUT_GENERIC>;
+        hipLaunchKernelGGL(pair, dim3(grid_for(2 * n, block)), dim3(block), 0,
+                           as_stream(stream), ext->ddev, bt, body_traj, (int64_t)nfull, t0, dt,
+                           w0, rel_t, (int64_t)n, tfinal, out);
+        return launch_status();
+    }
     auto kern = ev == EV_SCF  ? mockstream_kernel<true, LAYOUT_GENERIC>(nbody)
                 : ev == EV_MW ? mockstream_kernel<false, LAYOUT_MW>(nbody)
                               : mockstream_kernel<false, LAYOUT_GENERIC>(nbody);
-    const unsigned block = 64;
     hipLaunchKernelGGL(kern, dim3(grid_for(n, block)), dim3(block), 0, as_stream(stream),
                        ext->ddev, bt, body_traj, (int64_t)nfull, t0, dt, w0, rel_t, (int64_t)n,
                        tfinal, out);

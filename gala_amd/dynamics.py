@@ -468,6 +468,18 @@ class MockStreamGenerator:
         if t[1] < t[0]:
             w = w[:, ::-1]
             orbit_t = t[::-1]
+        # the bodies' forward trajectory on the full grid (mockstream.pyx:536-549),
+        # on a side stream (default priority), so that it runs while the DF is
+        # sampled (host work and the DF's own d2_dr2 launch)
+        dt = orbit_t[1] - orbit_t[0]
+        nfull = len(orbit_t)
+        t_ = dev.torch()
+        main = t_.cuda.current_stream()
+        side = t_.cuda.Stream()
+        side.wait_stream(main)
+        with t_.cuda.stream(side):
+            traj_dev, _ = nbody_leapfrog(pot, bodies, np.ascontiguousarray(w[:, 0, :]), dt,
+                                         nfull - 1, save=True)  # (6, nfull, K)
         prog_orbit = Orbit(pos=w[:3, :, 0], vel=w[3:, :, 0], t=orbit_t,
                            hamiltonian=self.hamiltonian)
         stream_w0 = self.df.sample(prog_orbit, prog_mass, hamiltonian=self.hamiltonian,
@@ -475,11 +487,8 @@ class MockStreamGenerator:
         # release bookkeeping (mockstream_generator.py:272-283)
         unq_t1s, nstream = np.unique(stream_w0.release_time, return_counts=True)
         all_nstream = release_counts(orbit_t, unq_t1s, nstream)
-        # the bodies' forward trajectory on the full grid (mockstream.pyx:536-549)
-        dt = orbit_t[1] - orbit_t[0]
-        nfull = len(orbit_t)
-        traj_dev, _ = nbody_leapfrog(pot, bodies, np.ascontiguousarray(w[:, 0, :]), dt,
-                                     nfull - 1, save=True)  # (6, nfull, K)
+        main.wait_stream(side)
+        traj_dev.record_stream(main)
         w0s = np.ascontiguousarray(stream_w0.w())
         n = w0s.shape[1]
         rel_t = np.asarray(stream_w0.release_time, dtype=np.float64)

@@ -356,3 +356,48 @@ def test_direct_nbody_dop853_default():
     ref, st, _ = O.dop853(mw_oracle(), prog_w0[:, None], t, nstiff=-1, nbatch=1)
     assert orbit.xyz.shape == (3, 301, 1)
     assert np.max(state_rel_err(orbit.w(), ref)) < 1e-8
+
+
+def test_one_body_lane_pairs_match_one_lane_kernel():
+    """A stream with the progenitor alone runs on lane pairs (progenitor copy
+    and particle side by side); with a second, potential-less body added the
+    one-lane kernel runs the same arithmetic for the progenitor and the
+    particles (the extra body pulls on nothing).  Both must agree bit for
+    bit, also when the progenitor has no potential."""
+    from gala_amd import _device as dev
+    from gala_amd import _lib
+    from gala_amd.integrate import leapfrog
+    from gala_amd.potential import MilkyWayLikePotential, PlummerPotential
+    prog_w0, prog_mass, b, dt, _ = workloads.mockstream_progenitor()
+    nfull = 601
+    ext = MilkyWayLikePotential()
+    prog_pot = PlummerPotential(m=prog_mass, b=b)
+    full_t = -600.0 + np.arange(nfull) * 1.0
+    traj, _ = leapfrog(ext, prog_w0[:, None], 1.0, nfull - 1, save=True)
+    traj1 = dev.to_host(traj)[:, :, 0]
+    rng = np.random.default_rng(5)
+    rel_idx = np.repeat(np.arange(0, nfull, 3), 2)
+    n = len(rel_idx)
+    w0 = traj1[:, rel_idx] + np.vstack([rng.normal(0, 0.05, (3, n)), rng.normal(0, 0.002, (3, n))])
+    rel_t = full_t[rel_idx]
+    # the extra body: somewhere far away, on the same grid
+    traj2 = np.stack([traj1, traj1 + np.array([50.0, -40.0, 30.0, 0, 0, 0])[:, None]], axis=2)
+    lib = _lib.load()
+    w0d, rtd = dev.to_dev(w0), dev.to_dev(rel_t)
+    print(f"{n} particles, {nfull} grid times", flush=True)
+
+    def run(handles, tr):
+        out = dev.empty((6, n))
+        trd = dev.to_dev(np.ascontiguousarray(tr))
+        _lib.check(lib.gamd_mockstream_leapfrog(ext.c_handle, _lib.body_array(handles),
+                                                len(handles), trd.data_ptr(), nfull, full_t[0],
+                                                1.0, w0d.data_ptr(), rtd.data_ptr(), n,
+                                                full_t[-1], out.data_ptr(), dev.stream_ptr()),
+                   "gamd_mockstream_leapfrog")
+        return dev.to_host(out)
+
+    for h in (prog_pot.c_handle, None):
+        pair = run([h], traj1)
+        one_lane = run([h, None], traj2)
+        assert np.all(np.isfinite(pair))
+        assert np.array_equal(pair, one_lane), h
