@@ -61,6 +61,12 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
                                  1e-10, 1e-10, 0, -1, 0.0, None, 0, None, None, None, None,
                                  None) == -1
     assert lib.gamd_potential_destroy(None) == 0
+    # the dense-output stream: null arguments, a zero window, a short workspace
+    assert lib.gamd_dop853_dense_workspace_bytes(10, 0) == -1
+    assert lib.gamd_dop853_dense_workspace_bytes(10, 3) == 256 + (2 * 6 * 3 + 79) * 10 * 8
+    assert lib.gamd_dop853_dense_stream(None, None, 1, None, None, None, 1e-8, 1e-10, 0, 0, 0.0,
+                                        None, 0, 1, 0, None, None, None, None, None, 0, 0, None,
+                                        0, None) == -1
 
 
 def test_batched_dop853_arguments_rejected():
@@ -102,6 +108,16 @@ def test_device_argument_checks_run_on_the_host():
                 np.array([-1, 0, 1]), torch.tensor([0, 2, 2])):
         with pytest.raises(ValueError):
             dev.check_permutation(bad, 3)
+    # the entry points check before they need a GPU
+    from gala_amd.integrate import dop853, dop853_to_host
+    t_out = np.linspace(0.0, 1.0, 5)
+    for fn, extra in ((dop853, {}), (dop853_to_host, {"t_out": t_out})):
+        with pytest.raises(ValueError):
+            fn(None, np.zeros((5, 4)), 0.0, 1.0, **extra)
+        with pytest.raises(ValueError):
+            fn(None, np.zeros((6, 4)), 0.0, np.ones(3), **extra)
+        with pytest.raises(ValueError):
+            fn(None, np.zeros((6, 3)), 0.0, 1.0, order=np.array([0, 0, 1]), **extra)
 
 
 def test_parse_time_specification():
