@@ -390,6 +390,21 @@ def release_counts(orbit_t, t1s, counts):
     return out
 
 
+_SIDE_STREAMS = {}
+
+
+def _side_stream():
+    """One extra stream per device for the whole process (default priority):
+    streams created per call cost far more than the work they overlap once
+    the process has made and released many of them."""
+    t = dev.torch()
+    d = t.cuda.current_device()
+    s = _SIDE_STREAMS.get(d)
+    if s is None:
+        s = _SIDE_STREAMS[d] = t.cuda.Stream()
+    return s
+
+
 class MockStreamGenerator:
     """Generate a mock stream (mockstream_generator.py:19-372).  Leapfrog
     runs the release-and-integrate loop in the HIP kernel
@@ -475,7 +490,7 @@ class MockStreamGenerator:
         nfull = len(orbit_t)
         t_ = dev.torch()
         main = t_.cuda.current_stream()
-        side = t_.cuda.Stream()
+        side = _side_stream()
         side.wait_stream(main)
         with t_.cuda.stream(side):
             traj_dev, _ = nbody_leapfrog(pot, bodies, np.ascontiguousarray(w[:, 0, :]), dt,
