@@ -395,8 +395,9 @@ _SIDE_STREAMS = {}
 
 def _side_stream():
     """One extra stream per device for the whole process (default priority):
-    streams created per call cost far more than the work they overlap once
-    the process has made and released many of them."""
+    a different stream on every call (torch hands them out in turn from its
+    pool) made the 8,002-particle stream take 44 ms end to end in a process
+    that had run other work first, against 7.8 ms with this one stream."""
     t = dev.torch()
     d = t.cuda.current_device()
     s = _SIDE_STREAMS.get(d)
