@@ -165,6 +165,10 @@ def bench_dop853(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded initial conditions, numpy default_rng(1))",
+        "metric_note": "value: exact FP64 orbit-steps/s (accepted DOP853 steps) of the "
+                       "1,048,576-orbit uneven workload; the exact-against-surrogate half of the "
+                       "metric is under scf_exact and scf_surrogate (value, speedup_vs_exact, "
+                       "dE_over_E)",
         "config": {"workload": "composite_dop853", "n_orbits": n,
                    "potential": "MW-like composite: Hernquist 1.7e9/0.07 + Hernquist 5e9/1.0 + "
                                 "Miyamoto-Nagai 6.8e10/3.0/0.28 + NFW 5.4e11/15.6",
