@@ -657,15 +657,18 @@ struct WaveBodies {
 // over the system's 6 nb components in the order of the (6, nb) state
 // vector, component-major, as dop853.cpp:416-444 and :463-486 loop over it:
 // every lane stages its six squared terms in LDS, rounded as the reference
-// rounds sqr*sqr, and every lane then adds all 6 nb terms in that order, so
-// the workgroup's lanes reach the same sums and take the same step decision.
-// Lanes past nb mirror orbit 0 of the batch and stay out of the sums.
+// rounds sqr*sqr, and the 6 nb terms of each sum are then added in that order
+// by one wave (a workgroup of two or more waves gives the two sums to two
+// waves, which halves the serial chain of dependent adds each wave waits on;
+// a one-wave batch adds both), so the workgroup's lanes reach the same sums
+// and take the same step decision.  Lanes past nb mirror orbit 0 of the batch
+// and stay out of the sums.
 template <bool SCF, int LAY>
 struct BatchLanes {
     static constexpr int NC = 6;
     int j, nb;
     int64_t base;  // the batch's first orbit
-    double *lds;   // 12 nb doubles: the a-terms, then the b-terms
+    double *lds;   // 12 nb + 2 doubles: the a-terms, the b-terms, the two sums
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
         orbit_rhs<SCF, LAY>(P, y, f);
     }
@@ -681,20 +684,36 @@ struct BatchLanes {
             }
         }
         __syncthreads();
+        if (nb > 64) {
+            // two or more waves: wave 0 adds the a-terms, wave 1 the b-terms,
+            // and they trade results through LDS
+            const int w = j >> 6;
+            if (w < 2) {
+                const double r = chain(w ? sb : sa, lds + w * n, n);
+                if ((j & 63) == 0) lds[2 * n + w] = r;
+            }
+            __syncthreads();
+            sa = lds[2 * n];
+            sb = lds[2 * n + 1];
+            return;
+        }
         double xa = sa, xb = sb;
         int i = 0;
-        for (; i + 4 <= n; i += 4) {
-            const double a0 = lds[i], a1 = lds[i + 1], a2 = lds[i + 2], a3 = lds[i + 3];
-            const double b0 = lds[n + i], b1 = lds[n + i + 1], b2 = lds[n + i + 2],
-                         b3 = lds[n + i + 3];
-            xa += a0;
-            xb += b0;
-            xa += a1;
-            xb += b1;
-            xa += a2;
-            xb += b2;
-            xa += a3;
-            xb += b3;
+        // eight terms of each sum read ahead of their adds (the adds keep the
+        // reference's order; the reads of a block overlap one LDS latency)
+        constexpr int U = 8;
+        for (; i + U <= n; i += U) {
+            double ta[U], tb[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                ta[u] = lds[i + u];
+                tb[u] = lds[n + i + u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                xa += ta[u];
+                xb += tb[u];
+            }
         }
         for (; i < n; i++) {
             xa += lds[i];
@@ -702,6 +721,20 @@ struct BatchLanes {
         }
         sa = xa;
         sb = xb;
+    }
+    // x + t[0] + t[1] + ... + t[n - 1], added in that order
+    __device__ __forceinline__ static double chain(double x, const double *t, int n) {
+        int i = 0;
+        constexpr int U = 8;
+        for (; i + U <= n; i += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = t[i + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) x += v[u];
+        }
+        for (; i < n; i++) x += t[i];
+        return x;
     }
     __device__ __forceinline__ double n() const { return 6.0 * (double)nb; }
     // dense sample s.oidx of the lane's orbit into w_out (6, n_out, N)
@@ -1553,7 +1586,7 @@ int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, c
                  hipStream_t s) {
     const int threads = (nbatch + 63) / 64 * 64;
     const int64_t grid = (a.N + nbatch - 1) / nbatch;
-    const size_t lds = (size_t)12 * nbatch * sizeof(double);
+    const size_t lds = (size_t)(12 * nbatch + 2) * sizeof(double);
     if (threads <= 256)
         hipLaunchKernelGGL((k_dop853_batch<SCF, DENSE, LAY, 256>), dim3((unsigned)grid),
                            dim3(threads), lds, s, P, nbatch, t0, t1, h0, a);
