@@ -238,7 +238,16 @@ def tune_dop853_dense(potential, w0, t0, t1, t_out, h0=None, rtol=1e-10, atol=1e
             return dict(e, cached=True)
     t = dev.torch()
     buf = t.empty((6, n_out, N), dtype=t.float64, pin_memory=True)
-    cands = sorted({chunk_rows_for(N, n_out, mb) for mb in mbs})
+    cands = {chunk_rows_for(N, n_out, mb) for mb in mbs}
+    # the whole grid as one window (every sample in HBM, one copy at the end)
+    # when its workspace takes at most half the free device memory: windows
+    # restart the persistent launch, and where the longest orbits set each
+    # window's time one launch can beat the overlapped copies
+    from . import _lib
+    whole = int(_lib.load().gamd_dop853_dense_workspace_bytes(N, n_out))
+    if whole <= t.cuda.mem_get_info()[0] // 2:
+        cands.add(n_out)
+    cands = sorted(cands)
     trials = {}
     for rows in cands:
         for graph in (False, True):

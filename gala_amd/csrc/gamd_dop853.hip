@@ -1536,49 +1536,146 @@ __global__ __launch_bounds__(64) void k_dop853_nbody(const DevPot *__restrict__ 
     if (t == 0) finish_stats(a, s, code);
 }
 
+// One launch of the batched kernel.  order (nullable): workgroup i takes
+// batch order[i] (entries outside [0, nbatches) are skipped).  st (nullable):
+// each batch's kept state, in the per-orbit window layout (WS_WORDS, N), every
+// orbit's column holding its own y and k1 and the batch's scalars; first: the
+// batches start from w0, else from st (a batch kept as finished is skipped).
+// pause > 0: a batch still stepping after that many attempts in this launch is
+// kept in st and left for the next launch.  Pausing between attempts changes
+// no step: a batch resumed from st takes the steps the one launch would.
+struct BatchRun {
+    const int64_t *order;
+    int64_t nbatches;
+    int64_t pause;
+    WinArgs w;
+};
+
 // dop853_integrate_hamiltonian with nbatch orbits per system (dop853.pyx:
 // 196-250): batch b, the orbits [b nbatch, min((b + 1) nbatch, N)), is one
 // (6, nb) system integrated by one workgroup (BatchLanes) from t0 to t1 with
 // initial step h0 (0: hinit over the batch) and dense output at t_out
 // (dop853_helper, :90-193).  The batch's status and step counts go to each of
 // its orbits.  Workgroups are independent: the hardware dispatcher hands
-// each free slot the next batch, which balances batches of uneven cost.
+// each free slot the next batch.
 template <bool SCF, bool DENSE, int LAY, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_dop853_batch(const DevPot *__restrict__ Pp, int nbatch,
-                                                      double t0, double t1, double h0, DopArgs a) {
+                                                      double t0, double t1, double h0, DopArgs a,
+                                                      BatchRun br) {
     extern __shared__ double lds[];
     const DevPot &P = *Pp;
     const int j = threadIdx.x;
-    const int64_t base = (int64_t)blockIdx.x * nbatch;
+    int64_t b = blockIdx.x;
+    if (br.order) {
+        b = br.order[b];
+        if (b < 0 || b >= br.nbatches) return;
+    }
+    const int64_t base = b * nbatch;
     const int64_t left = a.N - base;
     const int nb = left < (int64_t)nbatch ? (int)left : nbatch;
     const BatchLanes<SCF, LAY> sys{j, nb, base, lds};
     const int64_t orb = base + (j < nb ? j : 0);
     double y[6], k1[6];
-    FOR6 y[c] = a.w0[c * a.N + orb];
     DopState s;
-    s.orb = orb;
-    s.x = t0;
-    s.xend = t1;
-    s.h = h0;
-    s.hmax = a.hmax > 0.0 ? a.hmax : fabs(t1 - t0);  // dop853.cpp:787-788, 312
-    s.posneg = sign_d(1.0, t1 - t0);
-    s.facold = 1.0e-4;
-    s.hlamb = 0.0;
-    s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
-    s.nfcn = 2;
-    s.oidx = 0;
-    s.reject = s.iasti = s.nonsti = s.pend = 0;
-    sys.rhs(P, y, k1);
-    if (s.h == 0.0) s.h = hinit_batch<SCF, LAY>(P, sys, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+    if (br.w.st && !br.w.first) {
+        // every column of the batch was kept with the same flags
+        if (win_load(br.w, a.N, orb, s, y, k1) == 0) return;
+    } else {
+        FOR6 y[c] = a.w0[c * a.N + orb];
+        s.orb = orb;
+        s.x = t0;
+        s.xend = t1;
+        s.h = h0;
+        s.hmax = a.hmax > 0.0 ? a.hmax : fabs(t1 - t0);  // dop853.cpp:787-788, 312
+        s.posneg = sign_d(1.0, t1 - t0);
+        s.facold = 1.0e-4;
+        s.hlamb = 0.0;
+        s.nstep = s.naccpt = s.stiff = s.nrejct = 0;
+        s.nfcn = 2;
+        s.oidx = 0;
+        s.reject = s.iasti = s.nonsti = s.pend = 0;
+        sys.rhs(P, y, k1);
+        if (s.h == 0.0)
+            s.h = hinit_batch<SCF, LAY>(P, sys, y, s.posneg, k1, s.hmax, a.rtol, a.atol);
+    }
     int code = 0;
     // every attempt advances nstep and nmax bounds it; the step decisions are
     // uniform over the workgroup, so its lanes leave together
-    while (code == 0) code = dop_attempt<SCF, DENSE, LAY>(P, a, sys, s, y, k1);
-    if (j < nb) {
-        FOR6 a.w_final[c * a.N + orb] = y[c];
-        finish_stats(a, s, code);
+    for (int64_t n = br.pause > 0 ? br.pause : -1; code == 0 && n != 0; n--)
+        code = dop_attempt<SCF, DENSE, LAY>(P, a, sys, s, y, k1);
+    if (j >= nb) return;
+    if (code == 0) {  // paused: kept for the next launch
+        win_save_scalars(br.w, a.N, s, WF_RUN);
+        FOR6 win_save_comp(br.w, a.N, orb, c, y[c], k1[c]);
+        return;
     }
+    FOR6 a.w_final[c * a.N + orb] = y[c];
+    finish_stats(a, s, code);
+    if (br.w.st) win_save_scalars(br.w, a.N, s, 0);
+}
+
+// Predicted remaining attempts of each batch after a pause of `pause`
+// attempts from t0: pause (t1 - x) / (x - t0), +inf without progress, -1 for
+// a finished batch.
+__global__ void k_batch_keys(const double *__restrict__ st, int64_t N, int nbatch,
+                             int64_t nbatches, double t0, double t1, int64_t pause,
+                             double *__restrict__ key) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatches) return;
+    const int64_t col = b * nbatch;
+    const int64_t flags = reinterpret_cast<const int64_t *>(st)[WS_FLAGS * N + col];
+    const double x = st[WS_X * N + col];
+    double k = -1.0;
+    if (flags != 0) {
+        const double done = fabs(x - t0), rest = fabs(t1 - x);
+        k = done > 0.0 ? (double)pause * (rest / done) : INFINITY;
+        if (!(k >= 0.0)) k = INFINITY;  // NaN state: run it early, it will stop
+    }
+    key[b] = k;
+}
+
+// order = the batches by descending key, ties by index: batch b goes to
+// position #{keys above key[b]} + #{equal keys at lower index}.  Each
+// workgroup stages tiles of the keys in LDS.
+constexpr int RANK_TILE = 256;
+__global__ __launch_bounds__(RANK_TILE) void k_batch_rank(const double *__restrict__ key,
+                                                          int64_t n, int64_t *__restrict__ order) {
+    __shared__ double tile[RANK_TILE];
+    const int64_t b = (int64_t)blockIdx.x * RANK_TILE + threadIdx.x;
+    const double kb = b < n ? key[b] : 0.0;
+    int64_t rank = 0;
+    for (int64_t t = 0; t < n; t += RANK_TILE) {
+        __syncthreads();
+        if (t + threadIdx.x < n) tile[threadIdx.x] = key[t + threadIdx.x];
+        __syncthreads();
+        const int m = n - t < RANK_TILE ? (int)(n - t) : RANK_TILE;
+        for (int i = 0; i < m; i++) {
+            const double ki = tile[i];
+            rank += (ki > kb) | ((ki == kb) & (t + i < b));
+        }
+    }
+    if (b < n) order[rank] = b;
+}
+
+// Batches in two launches: the first takes each batch through a few attempts
+// and keeps it, the kept states rank the batches (longest predicted remainder
+// first, so the dispatcher does not leave the costliest batches to the end)
+// and the second resumes them in that order.  GAMD_DOP853_BATCH_ORDER: 0 =
+// one launch in index order, 1 = two launches whatever the batch count;
+// unset: two launches when there are more batches than resident workgroups.
+// GAMD_DOP853_BATCH_PILOT: the first launch's attempts per batch (default
+// 96).  Same results every way.
+int batch_order_mode() {
+    const char *env = getenv("GAMD_DOP853_BATCH_ORDER");
+    if (env && env[0] == '0') return 0;
+    if (env && env[0] == '1') return 1;
+    return -1;
+}
+
+int64_t batch_pilot() {
+    const char *env = getenv("GAMD_DOP853_BATCH_PILOT");
+    const int64_t v = env && env[0] ? atoll(env) : 0;
+    return v > 0 ? v : 96;
 }
 
 template <bool SCF, bool DENSE, int LAY>
@@ -1587,13 +1684,52 @@ int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, c
     const int threads = (nbatch + 63) / 64 * 64;
     const int64_t grid = (a.N + nbatch - 1) / nbatch;
     const size_t lds = (size_t)(12 * nbatch + 2) * sizeof(double);
-    if (threads <= 256)
-        hipLaunchKernelGGL((k_dop853_batch<SCF, DENSE, LAY, 256>), dim3((unsigned)grid),
-                           dim3(threads), lds, s, P, nbatch, t0, t1, h0, a);
-    else
-        hipLaunchKernelGGL((k_dop853_batch<SCF, DENSE, LAY, GAMD_DOP853_MAX_NBATCH>),
-                           dim3((unsigned)grid), dim3(threads), lds, s, P, nbatch, t0, t1, h0, a);
-    return launch_status();
+    const auto kern = threads <= 256 ? k_dop853_batch<SCF, DENSE, LAY, 256>
+                                     : k_dop853_batch<SCF, DENSE, LAY, GAMD_DOP853_MAX_NBATCH>;
+    BatchRun br{nullptr, grid, 0, WinArgs{nullptr, 1}};
+    const int mode = batch_order_mode();
+    const int64_t pilot = batch_pilot();
+    bool two = mode == 1;
+    if (mode < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) !=
+            hipSuccess)
+            per_cu = 0;
+        two = per_cu > 0 && grid > (int64_t)per_cu * cu_count();
+    }
+    double *st = nullptr;
+    if (two) {
+        // kept states, then keys and order
+        const size_t bytes = ((size_t)WS_WORDS * a.N + 2 * (size_t)grid) * sizeof(double);
+        if (hipMallocAsync((void **)&st, bytes, s) != hipSuccess) {
+            (void)hipGetLastError();
+            st = nullptr;
+        }
+    }
+    if (!st) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(threads), lds, s, P, nbatch, t0, t1,
+                           h0, a, br);
+        return launch_status();
+    }
+    double *key = st + (size_t)WS_WORDS * a.N;
+    int64_t *order = reinterpret_cast<int64_t *>(key + grid);
+    br.pause = pilot;
+    br.w.st = st;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(threads), lds, s, P, nbatch, t0, t1, h0,
+                       a, br);
+    const unsigned kb = (unsigned)((grid + 255) / 256);
+    hipLaunchKernelGGL(k_batch_keys, dim3(kb), dim3(256), 0, s, st, a.N, nbatch, grid, t0, t1,
+                       pilot, key);
+    const unsigned rb = (unsigned)((grid + RANK_TILE - 1) / RANK_TILE);
+    hipLaunchKernelGGL(k_batch_rank, dim3(rb), dim3(RANK_TILE), 0, s, key, grid, order);
+    br.order = order;
+    br.pause = 0;
+    br.w.first = 0;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(threads), lds, s, P, nbatch, t0, t1, h0,
+                       a, br);
+    const int rc = launch_status();
+    const hipError_t fe = hipFreeAsync(st, s);
+    return rc != GAMD_OK ? rc : (fe == hipSuccess ? GAMD_OK : GAMD_ERR_HIP);
 }
 
 // Systems of at most 4 bodies on one 16-lane row (two components per lane)

@@ -251,7 +251,12 @@ int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t
  * n_out) into w_out (6, n_out, N) when w_out != NULL.  w_final (6, N);
  * status (N) and stats (N, 4) (nullable) hold each orbit's batch's idid and
  * nstep, naccpt, nrejct, nfcn.  1 <= nbatch <= GAMD_DOP853_MAX_NBATCH
- * (GAMD_ERR_UNSUPPORTED above). */
+ * (GAMD_ERR_UNSUPPORTED above).  With more batches than the GPU holds at
+ * once, the batches run in two launches: every batch takes 96 attempts
+ * (GAMD_DOP853_BATCH_PILOT) and is kept, then the batches resume in order of
+ * their predicted remaining attempts, longest first; the kept states (29 N
+ * doubles) are allocated stream-ordered for the call.  Same results as one
+ * launch (GAMD_DOP853_BATCH_ORDER=0; =1 forces the two launches). */
 int gamd_dop853_batched(const gamd_potential *pot, const double *w0, size_t N, double t0,
                         double t1, double h0, double rtol, double atol, int64_t nmax,
                         int64_t nstiff, double hmax, int nbatch, const double *t_out,

@@ -155,3 +155,59 @@ def test_batched_failure_is_per_batch():
     with pytest.raises(RuntimeError):
         Hamiltonian(mw_product()).integrate_orbit(w0, t=t, Integrator="dopri853",
                                                   Integrator_kwargs=dict(nmax=2000))
+
+
+@pytest.mark.parametrize("nbatch,n,T,dense,pilot,nmax", [
+    (8, 4000, 300.0, False, 1, 0),       # pause after every batch's first attempt
+    (8, 4000, 300.0, True, 96, 0),       # dense output across the pause
+    (100, 2050, 300.0, True, 40, 0),     # two waves per batch, a ragged last batch
+    (37, 1000, 300.0, False, 100000, 0),  # every batch finishes in the first launch
+    (100, 1000, 500.0, False, 50, 400),  # a batch failing at nmax after the pause
+    (100, 1000, 500.0, False, 1000, 400),  # ... and before it
+])
+def test_ordered_launches_equal_one_launch(monkeypatch, nbatch, n, T, dense, pilot, nmax):
+    """Batches in two launches (a pause after `pilot` attempts, then the
+    batches resumed longest predicted remainder first) take exactly the steps
+    of one launch in index order: final states, dense output, statuses and
+    step counts bit for bit."""
+    from gala_amd import _device as dev
+    from gala_amd.integrate import dop853_batched
+    w0 = _ics(n, 11)
+    w0[:, 3] = [0.05, 0.0, 0.0, 0.0, 0.3, 0.0]  # a costly orbit in batch 0
+    t_out = np.linspace(0.0, T, 31) if dense else None
+    runs = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("GAMD_DOP853_BATCH_ORDER", mode)
+        monkeypatch.setenv("GAMD_DOP853_BATCH_PILOT", str(pilot))
+        r = dop853_batched(mw_product(), w0, 0.0, T, 1.0, nbatch=nbatch, rtol=1e-9, atol=1e-10,
+                           nmax=nmax, t_out=t_out)
+        runs.append({k: dev.to_host(v) for k, v in r.items()})
+    one, two = runs
+    st = one["status"]
+    print(f"nbatch={nbatch} n={n}: statuses {dict(zip(*np.unique(st, return_counts=True)))}, "
+          f"attempts per batch {one['stats'][::nbatch, 0].min()}-{one['stats'][::nbatch, 0].max()}")
+    if nmax:
+        assert np.any(st == -2)
+    for k in one:
+        assert np.array_equal(one[k], two[k], equal_nan=True), k
+
+
+def test_ordered_launches_by_default_for_many_batches(monkeypatch):
+    """With more batches than resident workgroups (20,000 orbits in batches
+    of 8: 2,500 one-wave workgroups) the two launches are the default; the
+    results equal one launch in index order bit for bit."""
+    from gala_amd import _device as dev
+    from gala_amd.integrate import dop853_batched
+    n = 20000
+    w0 = _ics(n, 12)
+    runs = []
+    for mode in ("0", None):
+        if mode is None:
+            monkeypatch.delenv("GAMD_DOP853_BATCH_ORDER", raising=False)
+        else:
+            monkeypatch.setenv("GAMD_DOP853_BATCH_ORDER", mode)
+        monkeypatch.delenv("GAMD_DOP853_BATCH_PILOT", raising=False)
+        r = dop853_batched(mw_product(), w0, 0.0, 200.0, 1.0, nbatch=8, rtol=1e-9, atol=1e-10)
+        runs.append({k: dev.to_host(v) for k, v in r.items()})
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
