@@ -1664,7 +1664,7 @@ __global__ __launch_bounds__(RANK_TILE) void k_batch_rank(const double *__restri
 // one launch in index order, 1 = two launches whatever the batch count;
 // unset: two launches when there are more batches than resident workgroups.
 // GAMD_DOP853_BATCH_PILOT: the first launch's attempts per batch (default
-// 96).  Same results every way.
+// 192).  Same results every way.
 int batch_order_mode() {
     const char *env = getenv("GAMD_DOP853_BATCH_ORDER");
     if (env && env[0] == '0') return 0;
@@ -1675,7 +1675,7 @@ int batch_order_mode() {
 int64_t batch_pilot() {
     const char *env = getenv("GAMD_DOP853_BATCH_PILOT");
     const int64_t v = env && env[0] ? atoll(env) : 0;
-    return v > 0 ? v : 96;
+    return v > 0 ? v : 192;
 }
 
 template <bool SCF, bool DENSE, int LAY>

@@ -252,7 +252,7 @@ int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t
  * status (N) and stats (N, 4) (nullable) hold each orbit's batch's idid and
  * nstep, naccpt, nrejct, nfcn.  1 <= nbatch <= GAMD_DOP853_MAX_NBATCH
  * (GAMD_ERR_UNSUPPORTED above).  With more batches than the GPU holds at
- * once, the batches run in two launches: every batch takes 96 attempts
+ * once, the batches run in two launches: every batch takes 192 attempts
  * (GAMD_DOP853_BATCH_PILOT) and is kept, then the batches resume in order of
  * their predicted remaining attempts, longest first; the kept states (29 N
  * doubles) are allocated stream-ordered for the call.  Same results as one
