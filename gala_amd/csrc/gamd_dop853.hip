@@ -1636,7 +1636,9 @@ __global__ void k_batch_keys(const double *__restrict__ st, int64_t N, int nbatc
 
 // order = the batches by descending key, ties by index: batch b goes to
 // position #{keys above key[b]} + #{equal keys at lower index}.  Each
-// workgroup stages tiles of the keys in LDS.
+// workgroup stages tiles of the keys in LDS; n^2 comparisons in all, which
+// for the batch counts it serves (one batch per 1 to 1,024 orbits, each
+// batch thousands of attempts) stays far below the integration's time.
 constexpr int RANK_TILE = 256;
 __global__ __launch_bounds__(RANK_TILE) void k_batch_rank(const double *__restrict__ key,
                                                           int64_t n, int64_t *__restrict__ order) {
@@ -1695,7 +1697,9 @@ int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, c
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) !=
             hipSuccess)
             per_cu = 0;
-        two = per_cu > 0 && grid > (int64_t)per_cu * cu_count();
+        // the ranking compares every pair of batches: at most 2^22 batches
+        // (~2 s of ranking against hours of integration)
+        two = per_cu > 0 && grid > (int64_t)per_cu * cu_count() && grid <= ((int64_t)1 << 22);
     }
     double *st = nullptr;
     if (two) {
