@@ -453,8 +453,8 @@ def bench_mockstream(args):
     2 particles per step (8002), each integrated forward to t = 0 with the
     progenitor's gravity; gala's default DOP853 generator and the Leapfrog
     one, end to end through MockStreamGenerator.run (host DF sampling
-    included), with the oracle's OpenMP release loop on the host beside the
-    Leapfrog kernel."""
+    included; the median of five calls after a warm-up), with the oracle's
+    OpenMP release loop on the host beside the Leapfrog kernel."""
     import torch
     from gala_amd.dynamics import FardalStreamDF, MockStreamGenerator
     from gala_amd.potential import MilkyWayLikePotential, PlummerPotential
@@ -474,17 +474,21 @@ def bench_mockstream(args):
             gens.append(gen)
             return gen.run(prog_w0, prog_mass, dt=dt, n_steps=nsteps, Integrator=integ)
         run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        stream, _ = run()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        walls = []
+        for _ in range(5):  # one call is milliseconds: the median of five
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            stream, _ = run()
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+        wall = float(np.median(walls))
         n = stream.xyz.shape[1]
         # particle-steps of the reference's fixed-step loop: release i runs
         # nsteps - i steps for each of its particles; each release's
         # progenitor copy steps alongside (mockstream.pyx:584-590)
         psteps = int(2 * sum(range(nsteps + 1)))
-        out[integ] = {"particles": n, "wall_s": wall,
+        out[integ] = {"particles": n, "wall_s": wall, "wall_s_min": min(walls),
+                      "wall_s_max": max(walls),
                       "particle_steps_per_s": psteps / wall if integ == "leapfrog" else None}
         ev = gens[-1].kernel_events
         if integ == "leapfrog" and ev is not None:
@@ -500,7 +504,8 @@ def bench_mockstream(args):
                 "kernel": "k_mockstream_pair (release-and-integrate, progenitor copy and particle on a lane pair)",
                 "achieved": flops / (kms / 1e3) / 1e12, "peak": roofline.PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": flops / (kms / 1e3) / 1e12 / roofline.PEAK_FP64_TFLOPS}
-        log(f"mock stream ({integ}): {n} particles, {wall * 1e3:.1f} ms end to end")
+        log(f"mock stream ({integ}): {n} particles, {wall * 1e3:.1f} ms end to end (median of "
+            f"5; {min(walls) * 1e3:.1f}-{max(walls) * 1e3:.1f})")
     if not args.no_cpu:
         from oracle import oracle as O
         rel_t = -np.arange(nsteps + 1)[::-1] * 1.0
