@@ -655,8 +655,8 @@ def main():
                     help="surrogate energy-drift bound for the exact-SCF fallback")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.exit("bench.py measures one GPU only: this build has no multi-GPU path "
-                 "(--gpus must be 1)")
+        sys.exit("bench.py measures one GPU only: gala_amd runs on one GPU and has no "
+                 "multi-GPU path (--gpus must be 1)")
     import torch
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
