@@ -211,3 +211,36 @@ def test_ordered_launches_by_default_for_many_batches(monkeypatch):
         runs.append({k: dev.to_host(v) for k, v in r.items()})
     for k in runs[0]:
         assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+@pytest.mark.parametrize("kind,nbatch,n,pilot,sign", [("generic", 300, 1400, 64, 1.0),
+                                                      ("generic", 100, 700, 30, -1.0),
+                                                      ("scf", 100, 300, 20, 1.0)])
+def test_ordered_launches_other_evaluators_and_wide_batches(monkeypatch, kind, nbatch, n, pilot,
+                                                           sign):
+    """The two launches with the generic component loop (a shifted
+    component), backward in time, with the SCF expansion, and with batches of
+    300 (five waves: the 1,024-thread kernel, three waves idle during the
+    sums), dense output included: bit for bit the one launch's results."""
+    from gala_amd import _device as dev, workloads
+    from gala_amd.integrate import dop853_batched
+    from gala_amd.potential import HernquistPotential, PlummerPotential, SCFPotential
+    if kind == "scf":
+        S, Tc = workloads.scf_coefficients(nmax=4, lmax=3, seed=2)
+        pot = SCFPotential(m=1e11, r_s=5.0, Snlm=S, Tnlm=Tc)
+    else:
+        pot = HernquistPotential(m=1e11, c=2.0) + PlummerPotential(m=1e10, b=1.0,
+                                                                   origin=[1.0, 0.5, -0.2])
+    w0 = _ics(n, 13)
+    t_out = sign * np.linspace(0.0, 400.0, 17)
+    runs = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("GAMD_DOP853_BATCH_ORDER", mode)
+        monkeypatch.setenv("GAMD_DOP853_BATCH_PILOT", str(pilot))
+        r = dop853_batched(pot, w0, 0.0, sign * 400.0, sign * 1.0, nbatch=nbatch, rtol=1e-9,
+                           atol=1e-10, t_out=t_out)
+        runs.append({k: dev.to_host(v) for k, v in r.items()})
+    assert np.all(runs[0]["status"] == 1)
+    assert runs[0]["stats"][:, 0].max() > pilot  # batches resumed in the second launch
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
