@@ -256,7 +256,8 @@ int gamd_dop853_dense_stream(const gamd_potential *pot, const double *w0, size_t
  * (GAMD_DOP853_BATCH_PILOT) and is kept, then the batches resume in order of
  * their predicted remaining attempts, longest first; the kept states (29 N
  * doubles) are allocated stream-ordered for the call.  Same results as one
- * launch (GAMD_DOP853_BATCH_ORDER=0; =1 forces the two launches). */
+ * launch (GAMD_DOP853_BATCH_ORDER=0, which allocates nothing; =1 forces the
+ * two launches). */
 int gamd_dop853_batched(const gamd_potential *pot, const double *w0, size_t N, double t0,
                         double t1, double h0, double rtol, double atol, int64_t nmax,
                         int64_t nstiff, double hmax, int nbatch, const double *t_out,
