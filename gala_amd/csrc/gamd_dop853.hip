@@ -666,9 +666,11 @@ struct WaveBodies {
 template <bool SCF, int LAY>
 struct BatchLanes {
     static constexpr int NC = 6;
+    static constexpr bool LDS_TABLEAU = true;
     int j, nb;
-    int64_t base;  // the batch's first orbit
-    double *lds;   // 12 nb + 2 doubles: the a-terms, the b-terms, the two sums
+    int64_t base;       // the batch's first orbit
+    double *lds;        // 12 nb + 2 doubles: the a-terms, the b-terms, the two sums
+    const double *tab;  // the tableau's LDS copy (TAB_WORDS doubles)
     __device__ __forceinline__ void rhs(const DevPot &P, const double (&y)[6], double (&f)[6]) const {
         orbit_rhs<SCF, LAY>(P, y, f);
     }
@@ -809,6 +811,44 @@ struct has_windows<S, decltype((void)S::WINDOWS)> {
     static constexpr bool value = S::WINDOWS;
 };
 
+// The tableau as one array (A 16x16, D 4x16, B, E5, BHH), for layouts that
+// read it from LDS (a layout declares LDS_TABLEAU = true and a member `tab`
+// pointing at an LDS copy of kTableau).
+constexpr int TAB_OFS_A = 0, TAB_OFS_D = 256, TAB_OFS_B = 320, TAB_OFS_E5 = 332,
+              TAB_OFS_BHH = 345, TAB_WORDS = 358;
+struct TableauWords {
+    double w[TAB_WORDS];
+};
+constexpr TableauWords make_tableau() {
+    TableauWords t{};
+    for (int i = 0; i < 16; i++)
+        for (int j = 0; j < 16; j++) t.w[TAB_OFS_A + i * 16 + j] = T::A[i][j];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 16; j++) t.w[TAB_OFS_D + i * 16 + j] = T::D[i][j];
+    for (int k = 0; k < 12; k++) t.w[TAB_OFS_B + k] = T::B[k];
+    for (int k = 0; k < 13; k++) t.w[TAB_OFS_E5 + k] = T::E5[k];
+    for (int k = 0; k < 13; k++) t.w[TAB_OFS_BHH + k] = T::BHH[k];
+    return t;
+}
+__constant__ TableauWords kTableau = make_tableau();
+
+template <class S, class = void>
+struct lds_tableau {
+    static constexpr bool value = false;
+};
+template <class S>
+struct lds_tableau<S, decltype((void)S::LDS_TABLEAU)> {
+    static constexpr bool value = S::LDS_TABLEAU;
+};
+
+template <class Sys>
+__device__ __forceinline__ double tab_entry(const Sys &sys, int ofs, int k, double c) {
+    if constexpr (lds_tableau<Sys>::value)
+        return sys.tab[ofs + k];
+    else
+        return c;
+}
+
 // One step attempt (dopcor basic step, dop853.cpp:339-648) for the lane's NC
 // components.  Returns 0 to go on, else the orbit's final idid.
 template <bool SCF, bool DENSE, int LAY, class Sys>
@@ -816,6 +856,17 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                                            DopState &s, double (&y)[Sys::NC],
                                            double (&k1)[Sys::NC]) {
     constexpr int NC = Sys::NC;
+    // tableau entries: compile-time constants, or for a layout that keeps the
+    // tableau in LDS, reads at an offset the compiler cannot move out of the
+    // attempt loop (held across the loop, the constants crowd the scalar
+    // registers and spill)
+    int tofs = 0;
+    if constexpr (lds_tableau<Sys>::value) asm volatile("" : "+v"(tofs));
+#define TAB_A(i, j) tab_entry(sys, tofs, TAB_OFS_A + (i) * 16 + (j), T::A[i][j])
+#define TAB_D(i, j) tab_entry(sys, tofs, TAB_OFS_D + (i) * 16 + (j), T::D[i][j])
+#define TAB_B(k) tab_entry(sys, tofs, TAB_OFS_B + (k), T::B[k])
+#define TAB_E5(k) tab_entry(sys, tofs, TAB_OFS_E5 + (k), T::E5[k])
+#define TAB_BHH(k) tab_entry(sys, tofs, TAB_OFS_BHH + (k), T::BHH[k])
     const double safe = 0.9, facc1 = 1.0 / 0.333, facc2 = 1.0 / 6.0;  // expo1 = 1/8
     if (s.nstep > a.nmax) return -2;
     if (0.1 * fabs(s.h) <= fabs(s.x) * a.uround) return -3;
@@ -824,42 +875,42 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
     s.h = h;
     s.nstep++;
     double k2[NC], k3[NC], k4[NC], k5[NC], k6[NC], k7[NC], k8[NC], k9[NC], k10[NC], yy[NC];
-    FORN yy[c] = y[c] + h * T::A[1][0] * k1[c];
+    FORN yy[c] = y[c] + h * TAB_A(1, 0) * k1[c];
     sys.rhs(P, yy, k2);
-    FORN yy[c] = y[c] + h * (T::A[2][0] * k1[c] + T::A[2][1] * k2[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(2, 0) * k1[c] + TAB_A(2, 1) * k2[c]);
     sys.rhs(P, yy, k3);
-    FORN yy[c] = y[c] + h * (T::A[3][0] * k1[c] + T::A[3][2] * k3[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(3, 0) * k1[c] + TAB_A(3, 2) * k3[c]);
     sys.rhs(P, yy, k4);
-    FORN yy[c] = y[c] + h * (T::A[4][0] * k1[c] + T::A[4][2] * k3[c] + T::A[4][3] * k4[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(4, 0) * k1[c] + TAB_A(4, 2) * k3[c] + TAB_A(4, 3) * k4[c]);
     sys.rhs(P, yy, k5);
-    FORN yy[c] = y[c] + h * (T::A[5][0] * k1[c] + T::A[5][3] * k4[c] + T::A[5][4] * k5[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(5, 0) * k1[c] + TAB_A(5, 3) * k4[c] + TAB_A(5, 4) * k5[c]);
     sys.rhs(P, yy, k6);
-    FORN yy[c] = y[c] + h * (T::A[6][0] * k1[c] + T::A[6][3] * k4[c] + T::A[6][4] * k5[c] +
-                             T::A[6][5] * k6[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(6, 0) * k1[c] + TAB_A(6, 3) * k4[c] + TAB_A(6, 4) * k5[c] +
+                             TAB_A(6, 5) * k6[c]);
     sys.rhs(P, yy, k7);
-    FORN yy[c] = y[c] + h * (T::A[7][0] * k1[c] + T::A[7][3] * k4[c] + T::A[7][4] * k5[c] +
-                             T::A[7][5] * k6[c] + T::A[7][6] * k7[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(7, 0) * k1[c] + TAB_A(7, 3) * k4[c] + TAB_A(7, 4) * k5[c] +
+                             TAB_A(7, 5) * k6[c] + TAB_A(7, 6) * k7[c]);
     sys.rhs(P, yy, k8);
-    FORN yy[c] = y[c] + h * (T::A[8][0] * k1[c] + T::A[8][3] * k4[c] + T::A[8][4] * k5[c] +
-                             T::A[8][5] * k6[c] + T::A[8][6] * k7[c] + T::A[8][7] * k8[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(8, 0) * k1[c] + TAB_A(8, 3) * k4[c] + TAB_A(8, 4) * k5[c] +
+                             TAB_A(8, 5) * k6[c] + TAB_A(8, 6) * k7[c] + TAB_A(8, 7) * k8[c]);
     sys.rhs(P, yy, k9);
-    FORN yy[c] = y[c] + h * (T::A[9][0] * k1[c] + T::A[9][3] * k4[c] + T::A[9][4] * k5[c] +
-                             T::A[9][5] * k6[c] + T::A[9][6] * k7[c] + T::A[9][7] * k8[c] +
-                             T::A[9][8] * k9[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(9, 0) * k1[c] + TAB_A(9, 3) * k4[c] + TAB_A(9, 4) * k5[c] +
+                             TAB_A(9, 5) * k6[c] + TAB_A(9, 6) * k7[c] + TAB_A(9, 7) * k8[c] +
+                             TAB_A(9, 8) * k9[c]);
     sys.rhs(P, yy, k10);
-    FORN yy[c] = y[c] + h * (T::A[10][0] * k1[c] + T::A[10][3] * k4[c] + T::A[10][4] * k5[c] +
-                             T::A[10][5] * k6[c] + T::A[10][6] * k7[c] + T::A[10][7] * k8[c] +
-                             T::A[10][8] * k9[c] + T::A[10][9] * k10[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(10, 0) * k1[c] + TAB_A(10, 3) * k4[c] + TAB_A(10, 4) * k5[c] +
+                             TAB_A(10, 5) * k6[c] + TAB_A(10, 6) * k7[c] + TAB_A(10, 7) * k8[c] +
+                             TAB_A(10, 8) * k9[c] + TAB_A(10, 9) * k10[c]);
     sys.rhs(P, yy, k2);  // stage 11 reuses k2 (dop853.cpp:402)
     const double xph = s.x + h;
-    FORN yy[c] = y[c] + h * (T::A[11][0] * k1[c] + T::A[11][3] * k4[c] + T::A[11][4] * k5[c] +
-                             T::A[11][5] * k6[c] + T::A[11][6] * k7[c] + T::A[11][7] * k8[c] +
-                             T::A[11][8] * k9[c] + T::A[11][9] * k10[c] + T::A[11][10] * k2[c]);
+    FORN yy[c] = y[c] + h * (TAB_A(11, 0) * k1[c] + TAB_A(11, 3) * k4[c] + TAB_A(11, 4) * k5[c] +
+                             TAB_A(11, 5) * k6[c] + TAB_A(11, 6) * k7[c] + TAB_A(11, 7) * k8[c] +
+                             TAB_A(11, 8) * k9[c] + TAB_A(11, 9) * k10[c] + TAB_A(11, 10) * k2[c]);
     sys.rhs(P, yy, k3);  // stage 12 reuses k3 (:408)
     s.nfcn += 11;
     FORN {
-        k4[c] = T::B[0] * k1[c] + T::B[5] * k6[c] + T::B[6] * k7[c] + T::B[7] * k8[c] +
-                T::B[8] * k9[c] + T::B[9] * k10[c] + T::B[10] * k2[c] + T::B[11] * k3[c];
+        k4[c] = TAB_B(0) * k1[c] + TAB_B(5) * k6[c] + TAB_B(6) * k7[c] + TAB_B(7) * k8[c] +
+                TAB_B(8) * k9[c] + TAB_B(9) * k10[c] + TAB_B(10) * k2[c] + TAB_B(11) * k3[c];
         k5[c] = y[c] + h * k4[c];
     }
     // error estimation (:416-444), n = 6; one divide serves both estimates
@@ -867,10 +918,10 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
     FORN {
         const double sk = a.atol + a.rtol * max_d(fabs(y[c]), fabs(k5[c]));
         const double isk = 1.0 / sk;
-        double erri = k4[c] - T::BHH[0] * k1[c] - T::BHH[8] * k9[c] - T::BHH[11] * k3[c];
+        double erri = k4[c] - TAB_BHH(0) * k1[c] - TAB_BHH(8) * k9[c] - TAB_BHH(11) * k3[c];
         e2[c] = erri * isk;
-        erri = T::E5[0] * k1[c] + T::E5[5] * k6[c] + T::E5[6] * k7[c] + T::E5[7] * k8[c] +
-               T::E5[8] * k9[c] + T::E5[9] * k10[c] + T::E5[10] * k2[c] + T::E5[11] * k3[c];
+        erri = TAB_E5(0) * k1[c] + TAB_E5(5) * k6[c] + TAB_E5(6) * k7[c] + TAB_E5(7) * k8[c] +
+               TAB_E5(8) * k9[c] + TAB_E5(9) * k10[c] + TAB_E5(10) * k2[c] + TAB_E5(11) * k3[c];
         e1[c] = erri * isk;
     }
     double err = 0.0, err2 = 0.0;
@@ -921,44 +972,44 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
                     const double bspl = h * k1[c] - ydiff;
                     r3[c] = bspl;
                     r4[c] = ydiff - h * k4[c] - bspl;
-                    r5[c] = T::D[0][0] * k1[c] + T::D[0][5] * k6[c] + T::D[0][6] * k7[c] +
-                            T::D[0][7] * k8[c] + T::D[0][8] * k9[c] + T::D[0][9] * k10[c] +
-                            T::D[0][10] * k2[c] + T::D[0][11] * k3[c];
-                    r6[c] = T::D[1][0] * k1[c] + T::D[1][5] * k6[c] + T::D[1][6] * k7[c] +
-                            T::D[1][7] * k8[c] + T::D[1][8] * k9[c] + T::D[1][9] * k10[c] +
-                            T::D[1][10] * k2[c] + T::D[1][11] * k3[c];
-                    r7[c] = T::D[2][0] * k1[c] + T::D[2][5] * k6[c] + T::D[2][6] * k7[c] +
-                            T::D[2][7] * k8[c] + T::D[2][8] * k9[c] + T::D[2][9] * k10[c] +
-                            T::D[2][10] * k2[c] + T::D[2][11] * k3[c];
-                    r8[c] = T::D[3][0] * k1[c] + T::D[3][5] * k6[c] + T::D[3][6] * k7[c] +
-                            T::D[3][7] * k8[c] + T::D[3][8] * k9[c] + T::D[3][9] * k10[c] +
-                            T::D[3][10] * k2[c] + T::D[3][11] * k3[c];
+                    r5[c] = TAB_D(0, 0) * k1[c] + TAB_D(0, 5) * k6[c] + TAB_D(0, 6) * k7[c] +
+                            TAB_D(0, 7) * k8[c] + TAB_D(0, 8) * k9[c] + TAB_D(0, 9) * k10[c] +
+                            TAB_D(0, 10) * k2[c] + TAB_D(0, 11) * k3[c];
+                    r6[c] = TAB_D(1, 0) * k1[c] + TAB_D(1, 5) * k6[c] + TAB_D(1, 6) * k7[c] +
+                            TAB_D(1, 7) * k8[c] + TAB_D(1, 8) * k9[c] + TAB_D(1, 9) * k10[c] +
+                            TAB_D(1, 10) * k2[c] + TAB_D(1, 11) * k3[c];
+                    r7[c] = TAB_D(2, 0) * k1[c] + TAB_D(2, 5) * k6[c] + TAB_D(2, 6) * k7[c] +
+                            TAB_D(2, 7) * k8[c] + TAB_D(2, 8) * k9[c] + TAB_D(2, 9) * k10[c] +
+                            TAB_D(2, 10) * k2[c] + TAB_D(2, 11) * k3[c];
+                    r8[c] = TAB_D(3, 0) * k1[c] + TAB_D(3, 5) * k6[c] + TAB_D(3, 6) * k7[c] +
+                            TAB_D(3, 7) * k8[c] + TAB_D(3, 8) * k9[c] + TAB_D(3, 9) * k10[c] +
+                            TAB_D(3, 10) * k2[c] + TAB_D(3, 11) * k3[c];
                 }
-                FORN yy[c] = y[c] + h * (T::A[13][0] * k1[c] + T::A[13][6] * k7[c] +
-                                         T::A[13][7] * k8[c] + T::A[13][8] * k9[c] +
-                                         T::A[13][9] * k10[c] + T::A[13][10] * k2[c] +
-                                         T::A[13][11] * k3[c] + T::A[13][12] * k4[c]);
+                FORN yy[c] = y[c] + h * (TAB_A(13, 0) * k1[c] + TAB_A(13, 6) * k7[c] +
+                                         TAB_A(13, 7) * k8[c] + TAB_A(13, 8) * k9[c] +
+                                         TAB_A(13, 9) * k10[c] + TAB_A(13, 10) * k2[c] +
+                                         TAB_A(13, 11) * k3[c] + TAB_A(13, 12) * k4[c]);
                 sys.rhs(P, yy, k10);
-                FORN yy[c] = y[c] + h * (T::A[14][0] * k1[c] + T::A[14][5] * k6[c] +
-                                         T::A[14][6] * k7[c] + T::A[14][7] * k8[c] +
-                                         T::A[14][10] * k2[c] + T::A[14][11] * k3[c] +
-                                         T::A[14][12] * k4[c] + T::A[14][13] * k10[c]);
+                FORN yy[c] = y[c] + h * (TAB_A(14, 0) * k1[c] + TAB_A(14, 5) * k6[c] +
+                                         TAB_A(14, 6) * k7[c] + TAB_A(14, 7) * k8[c] +
+                                         TAB_A(14, 10) * k2[c] + TAB_A(14, 11) * k3[c] +
+                                         TAB_A(14, 12) * k4[c] + TAB_A(14, 13) * k10[c]);
                 sys.rhs(P, yy, k2);
-                FORN yy[c] = y[c] + h * (T::A[15][0] * k1[c] + T::A[15][5] * k6[c] +
-                                         T::A[15][6] * k7[c] + T::A[15][7] * k8[c] +
-                                         T::A[15][8] * k9[c] + T::A[15][12] * k4[c] +
-                                         T::A[15][13] * k10[c] + T::A[15][14] * k2[c]);
+                FORN yy[c] = y[c] + h * (TAB_A(15, 0) * k1[c] + TAB_A(15, 5) * k6[c] +
+                                         TAB_A(15, 6) * k7[c] + TAB_A(15, 7) * k8[c] +
+                                         TAB_A(15, 8) * k9[c] + TAB_A(15, 12) * k4[c] +
+                                         TAB_A(15, 13) * k10[c] + TAB_A(15, 14) * k2[c]);
                 sys.rhs(P, yy, k3);
                 s.nfcn += 3;
                 FORN {
-                    r5[c] = h * (r5[c] + T::D[0][12] * k4[c] + T::D[0][13] * k10[c] +
-                                 T::D[0][14] * k2[c] + T::D[0][15] * k3[c]);
-                    r6[c] = h * (r6[c] + T::D[1][12] * k4[c] + T::D[1][13] * k10[c] +
-                                 T::D[1][14] * k2[c] + T::D[1][15] * k3[c]);
-                    r7[c] = h * (r7[c] + T::D[2][12] * k4[c] + T::D[2][13] * k10[c] +
-                                 T::D[2][14] * k2[c] + T::D[2][15] * k3[c]);
-                    r8[c] = h * (r8[c] + T::D[3][12] * k4[c] + T::D[3][13] * k10[c] +
-                                 T::D[3][14] * k2[c] + T::D[3][15] * k3[c]);
+                    r5[c] = h * (r5[c] + TAB_D(0, 12) * k4[c] + TAB_D(0, 13) * k10[c] +
+                                 TAB_D(0, 14) * k2[c] + TAB_D(0, 15) * k3[c]);
+                    r6[c] = h * (r6[c] + TAB_D(1, 12) * k4[c] + TAB_D(1, 13) * k10[c] +
+                                 TAB_D(1, 14) * k2[c] + TAB_D(1, 15) * k3[c]);
+                    r7[c] = h * (r7[c] + TAB_D(2, 12) * k4[c] + TAB_D(2, 13) * k10[c] +
+                                 TAB_D(2, 14) * k2[c] + TAB_D(2, 15) * k3[c]);
+                    r8[c] = h * (r8[c] + TAB_D(3, 12) * k4[c] + TAB_D(3, 13) * k10[c] +
+                                 TAB_D(3, 14) * k2[c] + TAB_D(3, 15) * k3[c]);
                 }
                 const double x0 = s.x, x1 = s.x + h;
                 while (s.oidx < a.out_end) {
@@ -1003,6 +1054,11 @@ __device__ __forceinline__ int dop_attempt(const DevPot &P, const DopArgs &a, co
     }
     if (code == 0) s.h = hnew;
     return code;
+#undef TAB_A
+#undef TAB_D
+#undef TAB_B
+#undef TAB_E5
+#undef TAB_BHH
 }
 
 __device__ __forceinline__ void finish_stats(const DopArgs &a, const DopState &s, int code) {
@@ -1573,7 +1629,10 @@ __global__ __launch_bounds__(MAXT) void k_dop853_batch(const DevPot *__restrict_
     const int64_t base = b * nbatch;
     const int64_t left = a.N - base;
     const int nb = left < (int64_t)nbatch ? (int)left : nbatch;
-    const BatchLanes<SCF, LAY> sys{j, nb, base, lds};
+    double *tab = lds + 12 * nbatch + 2;
+    for (int i = j; i < TAB_WORDS; i += blockDim.x) tab[i] = kTableau.w[i];
+    __syncthreads();
+    const BatchLanes<SCF, LAY> sys{j, nb, base, lds, tab};
     const int64_t orb = base + (j < nb ? j : 0);
     double y[6], k1[6];
     DopState s;
@@ -1685,7 +1744,7 @@ int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, c
                  hipStream_t s) {
     const int threads = (nbatch + 63) / 64 * 64;
     const int64_t grid = (a.N + nbatch - 1) / nbatch;
-    const size_t lds = (size_t)(12 * nbatch + 2) * sizeof(double);
+    const size_t lds = (size_t)(12 * nbatch + 2 + TAB_WORDS) * sizeof(double);
     const auto kern = threads <= 256 ? k_dop853_batch<SCF, DENSE, LAY, 256>
                                      : k_dop853_batch<SCF, DENSE, LAY, GAMD_DOP853_MAX_NBATCH>;
     BatchRun br{nullptr, grid, 0, WinArgs{nullptr, 1}};
