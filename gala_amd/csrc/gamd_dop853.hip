@@ -328,6 +328,13 @@ struct OneLane {
     }
 };
 
+// OneLane reading the tableau from an LDS copy (the queue loop's attempts)
+template <bool SCF, int LAY>
+struct OneLaneTab : OneLane<SCF, LAY> {
+    static constexpr bool LDS_TABLEAU = true;
+    const double *tab;
+};
+
 // one orbit per 16-lane row, lane c holding component comp
 template <bool SCF, int LAY>
 struct WideRow {
@@ -1327,6 +1334,9 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         y[c] = 0.0;
         k1[c] = 0.0;
     }
+    __shared__ double tab[TAB_WORDS];
+    for (int i = threadIdx.x; i < TAB_WORDS; i += blockDim.x) tab[i] = kTableau.w[i];
+    __syncthreads();
 
     if (a.n_tail > 0) {
         // the first waves of the grid take the queue order's first n_tail
@@ -1409,7 +1419,7 @@ __global__ __launch_bounds__(256) void k_dop853(const DevPot *__restrict__ Pp, D
         }
         if (!active) continue;
 
-        const int code = dop_attempt<SCF, DENSE, LAY>(P, a, OneLane<SCF, LAY>{}, s, y, k1);
+        const int code = dop_attempt<SCF, DENSE, LAY>(P, a, OneLaneTab<SCF, LAY>{{}, tab}, s, y, k1);
         if (code != 0) {
             FOR6 a.w_final[c * a.N + s.orb] = y[c];
             finish_stats(a, s, code);
