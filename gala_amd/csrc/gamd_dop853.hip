@@ -1767,7 +1767,8 @@ int launch_batch(const DevPot *P, int nbatch, double t0, double t1, double h0, c
             hipSuccess)
             per_cu = 0;
         // the ranking compares every pair of batches: at most 2^22 batches
-        // (~2 s of ranking against hours of integration)
+        // (~2 s of ranking there; integrating that many batches takes tens of
+        // seconds even at one orbit per batch)
         two = per_cu > 0 && grid > (int64_t)per_cu * cu_count() && grid <= ((int64_t)1 << 22);
     }
     double *st = nullptr;
