@@ -165,3 +165,26 @@ def test_integrate_orbit_many_orbits_per_orbit_dop853():
     ref = dop853(mw_product(), w0, t[0], t[-1], t[1] - t[0], t_out=t, **kw)
     assert np.all(dev.to_host(ref["status"]) == 1)
     assert np.array_equal(orbit.w(), dev.to_host(ref["w_out"]))
+
+
+def test_autotuner_tries_the_whole_grid_as_one_window(monkeypatch, tmp_path):
+    """The window autotuner times the whole t_out grid as one window beside
+    its chunk-sized candidates (when that workspace fits), remembers the
+    fastest, and dop853_to_host then uses it with the same samples as direct
+    writes."""
+    from gala_amd import autotune
+    from gala_amd.integrate import dop853, dop853_to_host
+    monkeypatch.setenv("GALA_AMD_AUTOTUNE_CACHE", str(tmp_path / "autotune.json"))
+    pot = mw_product()
+    w0, T = workloads.composite_dop853(vcirc_oracle(mw_oracle()), n=2000, seed=32)
+    t_out = np.linspace(0.0, 10000.0, 401)
+    # 8 and 32 MB of saved rows: windows of 83 and 333 samples, and the whole grid
+    tuned = autotune.tune_dop853_dense(pot, w0, 0.0, T, t_out, h0=1.0, rtol=1e-8, atol=1e-10,
+                                       mbs=(8, 32), repeats=1, force=True)
+    rows = sorted({int(k.split(",")[0].split("=")[1]) for k in tuned["trials_ms"]})
+    print(f"window candidates {rows}; picked {tuned['window_rows']} (graph {tuned['graph']})")
+    assert rows == [83, 333, 401]
+    assert tuned["window_rows"] in rows
+    direct = dop853(pot, w0, 0.0, T, 1.0, rtol=1e-8, atol=1e-10, t_out=t_out)
+    r = dop853_to_host(pot, w0, 0.0, T, t_out, h0=1.0, rtol=1e-8, atol=1e-10)
+    _check_equal(direct, r, _written(t_out, np.zeros_like(T), T), "tuned")
